@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the all-pairs CosineGreedy path on one MI355X.
+
+Default workload: 8192 x 8192 spectra (refs = queries, as the reference's own
+benchmark runs them), peaks round(lognormal(ln 60, 0.8)) clipped to
+[5, 1024], 20% of spectra with a dense cluster of 30-80 peaks inside 0.05 Da,
+m/z U[50, 1000] float32, intensities U(0, 1] scaled to max 1, tolerance 0.01,
+mz_power 0, intensity_power 1, match_limit 2048.  Inputs are resident in HBM
+before the timed region; one step is one full cs_cosine_greedy call (query
+tiling + lane kernel + cooperative kernel) producing score/matches/overflow
+for all 67,108,864 pairs.
+
+Prints one JSON line (rank 0).  ``--workload fingerprint`` runs the 4096 x
+4096 x 2048-bit Jaccard instead.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "cosine-similarity_amd"))
+
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 PMC counters (see
+    profiles/), or None when no measurement for this kernel/workload exists."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    e = t.get(f"{kernel}@{workload}")
+    return None if e is None else e.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline_cosine(packed, tol, seconds_target: float):
+    """The oracle (C restatement of matchms CosineGreedy, OpenMP over rows) on a
+    stripe of reference rows against all queries, on this host's cores."""
+    import numpy as np
+
+    from oracle import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    n = packed.n_spectra
+    csr = (packed.mz, packed.intensities, packed.offsets)
+
+    def stripe(rows):
+        off = packed.offsets
+        sub = (packed.mz[: off[rows]], packed.intensities[: off[rows]], off[: rows + 1])
+        t0 = time.perf_counter()
+        oracle.cosine_greedy_matrix(sub, csr, tolerance=tol, nthreads=cores)
+        return time.perf_counter() - t0
+
+    rows = 8
+    t = stripe(rows)
+    while t < 0.5 and rows < n:
+        rows = min(n, rows * 4)
+        t = stripe(rows)
+    rate = rows * n / t
+    want = int(min(n, max(rows, rate * seconds_target / n)))
+    if want > rows:
+        rows = want
+        t = stripe(rows)
+        rate = rows * n / t
+    return {"value": rate, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"first {rows} reference rows x all {n} queries ({rows * n} pairs), "
+                      f"{t:.1f} s, oracle/cs_oracle.c float64, OpenMP"}
+
+
+def bench_cosine(a) -> dict:
+    import numpy as np
+    import torch
+
+    from cudams import _native, synthetic
+    from cudams.device import Workspace, cosine_greedy
+    from cudams.spectra import DeviceSpectra
+
+    n, tol, ml = a.n, 0.01, 2048
+    log(f"generating {n} clustered lognormal spectra")
+    packed = synthetic.lognormal_spectra(n, 5)
+    dev = torch.device("cuda:0")
+    S = DeviceSpectra(packed, 0.0, 1.0, dev)
+    score = torch.empty((n, n), dtype=torch.float32, device=dev)
+    matches = torch.empty((n, n), dtype=torch.int32, device=dev)
+    overflow = torch.empty((n, n), dtype=torch.uint8, device=dev)
+    ws = Workspace(dev)
+
+    def step(stages=0):
+        cosine_greedy(S, S, tol, 0.0, ml, score, matches, overflow, ws, a.light_candidates,
+                      a.round_capacity, stages=stages)
+
+    log(f"warmup {a.warmup}")
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"timing {a.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / a.steps
+
+    # per-stage device time with HIP events on the stream the kernels run on
+    stream = torch.cuda.current_stream()
+    names = {_native.CS_STAGE_PREPARE: "prepare", _native.CS_STAGE_LIGHT: "light",
+             _native.CS_STAGE_HEAVY: "heavy"}
+    acc = {v: 0.0 for v in names.values()}
+    for _ in range(a.steps):
+        for st, name in names.items():
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            step(st)
+            e1.record(stream)
+            e1.synchronize()
+            acc[name] += e0.elapsed_time(e1)
+    stage_ms = {k: v / a.steps for k, v in acc.items()}
+    log(f"stages ms {stage_ms}")
+
+    lens = np.diff(packed.offsets).astype(np.float64)
+    pairs = n * n
+    alg_bytes = 8.0 * (n * lens.sum() + n * lens.sum()) + 9.0 * pairs
+    light_s = stage_ms["light"] / 1e3
+    achieved = alg_bytes / light_s / 1e9
+    res = {
+        "metric": "CosineGreedy pairs/s at 8k x 8k",
+        "value": pairs / (ms / 1e3),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cosine_greedy_8192x8192_lognormal_clusters", "refs": n,
+                   "queries": n, "tolerance": tol, "mz_power": 0.0, "intensity_power": 1.0,
+                   "match_limit": ml, "peaks": "round(lognormal(ln 60, 0.8)) in [5, 1024]",
+                   "cluster_fraction": 0.2, "seed": 5, "mean_peaks": float(lens.mean())},
+        "stage_ms": stage_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles("k_light", "cosine_8192")},
+    }
+    if not a.no_cpu_baseline:
+        log("cpu baseline")
+        res["cpu_baseline"] = cpu_baseline_cosine(packed, tol, a.cpu_seconds)
+    else:
+        res["cpu_baseline"] = None
+    return res
+
+
+def bench_fingerprint(a) -> dict:
+    import numpy as np
+    import torch
+
+    from cudams import synthetic
+    from cudams.device import Workspace, fingerprint_similarity
+
+    n, bits = 4096, 2048
+    dev = torch.device("cuda:0")
+    fa = synthetic.fingerprints(n, 3, n_bits=bits)
+    fb = synthetic.fingerprints(n, 4, n_bits=bits)
+    wa = torch.from_numpy(synthetic.pack_bits(fa).view(np.int32)).to(dev)
+    wb = torch.from_numpy(synthetic.pack_bits(fb).view(np.int32)).to(dev)
+    out = torch.empty((n, n), dtype=torch.float32, device=dev)
+    ws = Workspace(dev)
+    for _ in range(a.warmup):
+        fingerprint_similarity("jaccard", wa, wb, out, workspace=ws)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        fingerprint_similarity("jaccard", wa, wb, out, workspace=ws)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    ops = 2.0 * n * n * bits
+    res = {
+        "metric": "FingerprintSimilarity jaccard pairs/s at 4096 x 4096 x 2048 bits",
+        "value": n * n / (ms / 1e3), "unit": "pairs/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int8", "data": "synthetic",
+        "config": {"workload": "fingerprint_4096x4096x2048_jaccard"},
+        "roofline": {"bound": "mfma", "achieved": ops / (ms / 1e3) / 1e12, "peak": INT8_PEAK_TOPS,
+                     "unit": "TFLOP/s", "frac": ops / (ms / 1e3) / 1e12 / INT8_PEAK_TOPS,
+                     "traffic": None},
+        "cpu_baseline": None,
+    }
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--workload", choices=["cosine", "fingerprint"], default="cosine")
+    ap.add_argument("--light-candidates", type=int, default=0)
+    ap.add_argument("--round-capacity", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no "
+              "multi-GPU path to time.", file=sys.stderr)
+        return 2
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 1
+    res = bench_cosine(a) if a.workload == "cosine" else bench_fingerprint(a)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,174 @@
+// cg_lane.h -- per-pair CosineGreedy logic shared by the device kernels and
+// the host-side unit harness (tests/native/lane_harness.cpp).
+//
+// Everything here is __host__ __device__ and free of wave intrinsics, so the
+// exact code a GPU lane runs for one (reference, query) pair can also be run
+// on the CPU against the oracle.
+//
+// Semantics (matchms CosineGreedy.pair as the reference calls it,
+// cudams/similarity/spectrum_similarity_functions.py:306-317):
+//   candidate (i, j)  iff  lb_i <= mz_q[j] + shift <= hb_i  with
+//                    lb_i = mz_r[i] - tol, hb_i = mz_r[i] + tol in float64
+//                    (find_matches bounds; the reference GPU scan is :123-149)
+//   value            v = ((mz_pow_r[i] * int_pow_r[i]) * mz_pow_q[j]) * int_pow_q[j]
+//   order            v descending, ties: later-collected first.  Collection
+//                    order is (i, j) ascending, so the total order is
+//                    (v desc, i desc, j desc) -- stable ascending sort then
+//                    reverse (:314), which the reference GPU mergesort
+//                    reproduces by taking the right run on ties (:178-182).
+//   greedy           accept iff i and j are both unused (:208-218)
+//   score            sum of accepted v (in order) / (norm_r * norm_q) (:261)
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CS_HD __host__ __device__ inline __attribute__((always_inline))
+#else
+#define CS_HD inline
+#endif
+
+namespace cs {
+
+// Sort key of one candidate: (v, packed (i << 16) | j), compared
+// lexicographically.  Peak indices fit 16 bits (max_peaks <= 65535), so the
+// packed integer orders (i desc, j desc) exactly as the collection order does.
+CS_HD bool key_greater(double va, uint32_t ia, double vb, uint32_t ib) {
+    return va > vb || (va == vb && ia > ib);
+}
+
+CS_HD uint32_t pack_ij(int i, int j) { return (uint32_t(i) << 16) | uint32_t(j); }
+
+// float64 window test exactly as matchms find_matches writes it.
+CS_HD bool exact_window(double mz_r, double mz_q, double tol, double shift) {
+    const double x = mz_q + shift;
+    return !(x > mz_r + tol) && !(x < mz_r - tol);
+}
+
+// Up to K candidates kept sorted by key_greater (descending) in registers.
+// All indexing is compile-time after unrolling, so the arrays stay in VGPRs.
+template <int K>
+struct LaneCandidates {
+    double v[K];
+    uint32_t ij[K];
+    int n;
+
+    CS_HD void reset() { n = 0; }
+
+    // Insert (cv, cij); caller guarantees n < K.
+    CS_HD void insert(double cv, uint32_t cij) {
+#pragma unroll
+        for (int s = 0; s < K; ++s) {
+            const bool take = (s >= n) || key_greater(cv, cij, v[s], ij[s]);
+            const double tv = take ? v[s] : cv;
+            const uint32_t tij = take ? ij[s] : cij;
+            v[s] = take ? cv : v[s];
+            ij[s] = take ? cij : ij[s];
+            cv = tv;
+            cij = tij;
+        }
+        ++n;
+    }
+
+    // Greedy unique assignment over the sorted candidates.
+    CS_HD void greedy(double &score, int &matches) const {
+        double s = 0.0;
+        int m = 0;
+        bool acc[K];
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+            bool ok = a < n;
+            const uint32_t ia = ij[a] >> 16, ja = ij[a] & 0xffffu;
+#pragma unroll
+            for (int b = 0; b < a; ++b) {
+                const bool clash = ((ij[b] >> 16) == ia) || ((ij[b] & 0xffffu) == ja);
+                ok = ok && !(acc[b] && clash);
+            }
+            acc[a] = ok;
+            if (ok) {
+                s += v[a];
+                ++m;
+            }
+        }
+        score = s;
+        matches = m;
+    }
+};
+
+// Result of scanning one pair in a lane.
+struct LaneResult {
+    double score;     // un-normalised greedy sum
+    int matches;
+    int candidates;   // exact candidate count (capped at K + 1 when heavy)
+    bool heavy;       // more than K candidates: pair needs the cooperative path
+};
+
+// The lane scan: a merge over the reference (pointer lo .. k) and the query
+// (pointer j) with a float32 coarse window `tolc` that is wider than `tol` by
+// more than every rounding error involved, followed by the exact float64 test
+// on each coarse hit.
+//
+//   RefMz(k)       -> float   reference m/z (float32), k in [0, nr)
+//   QryMz(j)       -> float   query m/z + shift (float32), j in [0, nq];
+//                             must return +inf at j == nq
+//   Exact(k, j)    -> bool    float64 window test of (k, j)
+//   Value(k, j)    -> double  power product of (k, j)
+//
+// Each step advances exactly one of lo, j or k, so a pair costs at most
+// nr + nq + (#coarse hits) steps.
+template <int K, class RefMz, class QryMz, class Exact, class Value>
+CS_HD LaneResult scan_pair(int nr, int nq, float tolc, RefMz ref_mz, QryMz qry_mz, Exact exact,
+                           Value value, LaneCandidates<K> &cand) {
+    cand.reset();
+    LaneResult res{0.0, 0, 0, false};
+    if (nr <= 0 || nq <= 0) return res;
+    const float inf = __builtin_huge_valf();
+    int lo = 0, k = 0, j = 0, count = 0;
+    bool heavy = false;
+    float vk = ref_mz(0);
+    float vq = qry_mz(0);
+    bool active = true;
+    while (active) {
+        const float d = vk - vq;
+        const bool below = d < -tolc;  // reference peak under the query window (k == lo)
+        const bool above = d > tolc;   // reference peak over it (or k == nr)
+        if (!below && !above) {
+            if (exact(k, j)) {
+                if (count < K) {
+                    cand.insert(value(k, j), pack_ij(k, j));
+                } else {
+                    heavy = true;
+                }
+                ++count;
+            }
+        }
+        lo = below ? lo + 1 : lo;
+        k = (below || above) ? lo : k + 1;
+        j = above ? j + 1 : j;
+        const float t = ref_mz(k < nr ? k : nr - 1);
+        vk = k < nr ? t : inf;
+        vq = qry_mz(j);
+        active = (j < nq) && (lo < nr) && !heavy;
+    }
+    res.candidates = count;
+    res.heavy = heavy;
+    if (!heavy) cand.greedy(res.score, res.matches);
+    return res;
+}
+
+// Coarse float32 tolerance covering every rounding error of the float32 scan
+// relative to the float64 test: m/z rounding of both peaks, the shift and its
+// float32 sum, the float32 difference, and the float64 rounding of
+// mz_r +- tol and mz_q + shift.  Each is at most 2^-24 of (max|mz| + |shift| +
+// tol); 8 of them bound the total, and the result is rounded up.
+inline float coarse_tolerance(double tol, double shift, double max_abs_mz) {
+    const double m = (max_abs_mz < 0 ? -max_abs_mz : max_abs_mz) + (shift < 0 ? -shift : shift) + tol;
+    const double e = 8.0 * m / 16777216.0;
+    const double t = tol + e;
+    float f = (float)t;
+    if ((double)f < t) f = __builtin_nextafterf(f, __builtin_huge_valf());
+    return __builtin_nextafterf(f, __builtin_huge_valf());
+}
+
+}  // namespace cs

@@ -1,0 +1,115 @@
+"""ctypes binding of libcsamd.so (the gfx950 HIP library, C ABI in
+include/cosine_similarity_amd.h).
+
+There is no fallback: if the library is missing or the machine has no ROCm
+GPU, every compute call raises.  Build it with
+``python cosine-similarity_amd/build.py``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcsamd.so")
+
+CS_OK = 0
+CS_ERR_INVALID = 1
+CS_ERR_WORKSPACE = 2
+CS_ERR_HIP = 3
+CS_ERR_UNSUPPORTED = 4
+
+CS_JACCARD, CS_DICE, CS_COSINE = 0, 1, 2
+
+# Every symbol include/cosine_similarity_amd.h declares.
+EXPORTED_SYMBOLS = (
+    "cs_last_error",
+    "cs_version",
+    "cs_cosine_greedy_workspace_size",
+    "cs_cosine_greedy",
+    "cs_fingerprint_workspace_size",
+    "cs_fingerprint_similarity",
+)
+
+
+class CsSpectra(ctypes.Structure):
+    """struct cs_spectra."""
+
+    _fields_ = [
+        ("n_spectra", ctypes.c_int64),
+        ("n_peaks", ctypes.c_int64),
+        ("offsets", ctypes.c_void_p),
+        ("mz", ctypes.c_void_p),
+        ("mz_exact", ctypes.c_void_p),
+        ("mz_pow", ctypes.c_void_p),
+        ("int_pow", ctypes.c_void_p),
+        ("norm", ctypes.c_void_p),
+        ("max_peaks", ctypes.c_int32),
+        ("max_abs_mz", ctypes.c_float),
+    ]
+
+
+class CsCosineParams(ctypes.Structure):
+    """struct cs_cosine_params."""
+
+    _fields_ = [
+        ("tolerance", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("match_limit", ctypes.c_int64),
+        ("light_candidates", ctypes.c_int32),
+        ("round_capacity", ctypes.c_int32),
+        ("stages", ctypes.c_int32),
+    ]
+
+
+CS_STAGE_PREPARE, CS_STAGE_LIGHT, CS_STAGE_HEAVY = 1, 2, 4
+
+
+class NativeError(RuntimeError):
+    """A cs_* call returned a non-zero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"libcsamd status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libcsamd.so (once) and declare the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is not built; run `python cosine-similarity_amd/build.py` "
+            "(hipcc --offload-arch=gfx950). There is no CPU fallback.")
+    L = ctypes.CDLL(path)
+    p, i64, i32, sz = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_size_t
+    L.cs_last_error.restype = ctypes.c_char_p
+    L.cs_last_error.argtypes = []
+    L.cs_version.restype = ctypes.c_int
+    L.cs_version.argtypes = []
+    L.cs_cosine_greedy_workspace_size.restype = i32
+    L.cs_cosine_greedy_workspace_size.argtypes = [ctypes.POINTER(CsSpectra),
+                                                  ctypes.POINTER(CsSpectra),
+                                                  ctypes.POINTER(sz)]
+    L.cs_cosine_greedy.restype = i32
+    L.cs_cosine_greedy.argtypes = [ctypes.POINTER(CsSpectra), ctypes.POINTER(CsSpectra),
+                                   ctypes.POINTER(CsCosineParams), p, p, p, i64, p, sz, p]
+    L.cs_fingerprint_workspace_size.restype = i32
+    L.cs_fingerprint_workspace_size.argtypes = [i64, i64, i64, ctypes.POINTER(sz)]
+    L.cs_fingerprint_similarity.restype = i32
+    L.cs_fingerprint_similarity.argtypes = [i32, p, i64, p, i64, i64, p, p, ctypes.c_float, p,
+                                            p, i64, p, sz, p]
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != CS_OK:
+        msg = load().cs_last_error().decode(errors="replace")
+        if status == CS_ERR_INVALID:
+            raise ValueError(f"libcsamd: {msg}")
+        raise NativeError(status, msg)

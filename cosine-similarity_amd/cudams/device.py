@@ -1,0 +1,115 @@
+"""Device-resident entry points: thin wrappers over the C ABI.
+
+Inputs and outputs are torch tensors already on the GPU; calls are
+asynchronous on torch's current stream.  The plug-in classes, the batched-job
+runner, the server and bench.py all go through here.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from . import _native
+from .spectra import DeviceSpectra
+
+
+class Workspace:
+    """A grow-only device byte buffer reused across calls on one stream."""
+
+    def __init__(self, device="cuda"):
+        self.device = device
+        self.buf: Optional[torch.Tensor] = None
+
+    def get(self, nbytes: int) -> torch.Tensor:
+        nbytes = max(int(nbytes), 256)
+        if self.buf is None or self.buf.numel() < nbytes:
+            self.buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        return self.buf
+
+
+def _stream_ptr(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("no ROCm GPU is visible; the HIP library has no CPU path")
+
+
+def cosine_greedy_workspace_bytes(refs: DeviceSpectra, queries: DeviceSpectra) -> int:
+    L = _native.load()
+    n = ctypes.c_size_t(0)
+    rs, qs = refs.struct(), queries.struct()
+    _native.check(L.cs_cosine_greedy_workspace_size(ctypes.byref(rs), ctypes.byref(qs),
+                                                    ctypes.byref(n)))
+    return int(n.value)
+
+
+def cosine_greedy(refs: DeviceSpectra, queries: DeviceSpectra, tolerance: float, shift: float,
+                  match_limit: int, score: torch.Tensor, matches: torch.Tensor,
+                  overflow: torch.Tensor, workspace: Optional[Workspace] = None,
+                  light_candidates: int = 0, round_capacity: int = 0, stream=None,
+                  stages: int = 0) -> None:
+    """All pairs into row-major [R, ld] outputs (float32, int32, uint8).
+    ``stages`` (a mask of _native.CS_STAGE_*) runs part of the pipeline."""
+    require_gpu()
+    L = _native.load()
+    R, Q = refs.n_spectra, queries.n_spectra
+    for t, dt in ((score, torch.float32), (matches, torch.int32), (overflow, torch.uint8)):
+        if t.dtype != dt or not t.is_cuda or t.dim() != 2 or t.shape[0] < R or t.shape[1] < Q:
+            raise ValueError(f"output must be a CUDA {dt} tensor of shape >= ({R}, {Q})")
+        if t.stride(1) != 1 or t.stride(0) != score.stride(0):
+            raise ValueError("outputs must be row-major with one common row stride")
+    ws = workspace or Workspace(score.device)
+    nbytes = cosine_greedy_workspace_bytes(refs, queries)
+    buf = ws.get(nbytes)
+    rs, qs = refs.struct(), queries.struct()
+    p = _native.CsCosineParams(float(tolerance), float(shift), int(match_limit),
+                               int(light_candidates), int(round_capacity), int(stages))
+    _native.check(L.cs_cosine_greedy(ctypes.byref(rs), ctypes.byref(qs), ctypes.byref(p),
+                                     score.data_ptr(), matches.data_ptr(), overflow.data_ptr(),
+                                     score.stride(0), buf.data_ptr(), buf.numel(),
+                                     _stream_ptr(stream)))
+
+
+MEASURES = {"jaccard": _native.CS_JACCARD, "dice": _native.CS_DICE, "cosine": _native.CS_COSINE}
+
+
+def fingerprint_similarity(measure: str, ref_words: torch.Tensor, query_words: torch.Tensor,
+                           out: torch.Tensor, ref_valid: Optional[torch.Tensor] = None,
+                           query_valid: Optional[torch.Tensor] = None,
+                           empty_score: float = float("nan"),
+                           intersections: Optional[torch.Tensor] = None,
+                           workspace: Optional[Workspace] = None, stream=None) -> None:
+    """out[r, q] = measure between bit-packed fingerprints (uint32 words as int32)."""
+    require_gpu()
+    L = _native.load()
+    if measure not in MEASURES:
+        raise ValueError(f"unknown similarity measure {measure!r}")
+    R, W = ref_words.shape
+    Q, W2 = query_words.shape
+    if W != W2:
+        raise ValueError("fingerprints differ in length")
+    if not (ref_words.is_contiguous() and query_words.is_contiguous()):
+        raise ValueError("fingerprint word arrays must be contiguous")
+    if out.dtype != torch.float32 or out.shape[0] < R or out.shape[1] < Q or out.stride(1) != 1:
+        raise ValueError("out must be a row-major float32 tensor of shape >= (R, Q)")
+    ws = workspace or Workspace(out.device)
+    n = ctypes.c_size_t(0)
+    _native.check(L.cs_fingerprint_workspace_size(R, Q, W, ctypes.byref(n)))
+    buf = ws.get(n.value)
+
+    def ptr(t):
+        return None if t is None else t.data_ptr()
+
+    if intersections is not None and (intersections.dtype != torch.int32 or
+                                      intersections.stride(0) != out.stride(0)):
+        raise ValueError("intersections must be int32 with the row stride of out")
+    _native.check(L.cs_fingerprint_similarity(
+        MEASURES[measure], ref_words.data_ptr(), R,
+        query_words.data_ptr(), Q, W, ptr(ref_valid), ptr(query_valid),
+        float(empty_score), out.data_ptr(), ptr(intersections), out.stride(0), buf.data_ptr(),
+        buf.numel(), _stream_ptr(stream)))

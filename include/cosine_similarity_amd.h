@@ -1,0 +1,159 @@
+/*
+ * cosine_similarity_amd.h -- C ABI of the MI355X (gfx950) spectral-similarity
+ * library (libcsamd.so).
+ *
+ * Plain C: device pointers, sizes and an opaque stream.  No exceptions cross
+ * this boundary; every entry point returns a cs_status and leaves a message
+ * for cs_last_error() on failure.  The caller owns every buffer (inputs,
+ * outputs, workspace); the library allocates no device memory and never
+ * synchronises the stream.  Calls on different streams are independent.
+ *
+ * Reference interfaces replaced (files in tornikeo/cosine-similarity):
+ *   - the Numba-CUDA launcher `kernel(rspec, qspec, lens, out, stream)` built
+ *     by cosine_greedy_kernel(), cudams/similarity/spectrum_similarity_
+ *     functions.py:12-37 (factory) and :264-276 (launcher), called from
+ *     cudams/similarity/CudaCosineGreedy.py:179;
+ *   - the torch implementations of jaccard/dice/cosine_similarity_matrix,
+ *     cudams/similarity/vector_similarity_functions.py:12-47, :50-91, :94-119,
+ *     called from cudams/similarity/CudaFingerprintSimilarity.py:166-177.
+ */
+#ifndef COSINE_SIMILARITY_AMD_H
+#define COSINE_SIMILARITY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cs_status {
+    CS_OK = 0,
+    CS_ERR_INVALID = 1,     /* bad argument (null pointer, size, layout) */
+    CS_ERR_WORKSPACE = 2,   /* workspace smaller than the *_workspace_size result */
+    CS_ERR_HIP = 3,         /* a HIP runtime call failed (launch, attribute) */
+    CS_ERR_UNSUPPORTED = 4  /* input outside the supported envelope */
+} cs_status;
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char *cs_last_error(void);
+
+/* Library ABI version (major * 10000 + minor * 100 + patch). */
+int cs_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Spectra                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * A set of spectra resident in device memory as ragged CSR.  Spectrum s owns
+ * peaks [offsets[s], offsets[s+1]) of the peak arrays; its m/z are ascending.
+ * offsets[0] need not be 0 (a slice of a larger set), and n_peaks must be at
+ * least offsets[n_spectra] - offsets[0].  Replaces the padded
+ * [2, batch, n_max_peaks] float32 tensor + lens of CudaCosineGreedy.py:63-87.
+ *
+ * The per-peak power factors are computed by the caller in float64 exactly as
+ * matchms writes them (x ** power), so the products the library forms,
+ * ((mz_pow_r * int_pow_r) * mz_pow_q) * int_pow_q, are bit-identical to
+ * matchms collect_peak_pairs.
+ */
+typedef struct cs_spectra {
+    int64_t n_spectra;
+    int64_t n_peaks;
+    const int64_t *offsets;  /* device [n_spectra + 1]                             */
+    const float *mz;         /* device [n_peaks] m/z rounded to nearest float32   */
+    const double *mz_exact;  /* device [n_peaks] m/z as given (float64); NULL when
+                                every value of `mz` is exact                      */
+    const double *mz_pow;    /* device [n_peaks] m/z ** mz_power; NULL means 1.0
+                                (mz_power == 0)                                   */
+    const double *int_pow;   /* device [n_peaks] intensity ** intensity_power     */
+    const double *norm;      /* device [n_spectra] sqrt(sum((mz_pow*int_pow)^2))  */
+    int32_t max_peaks;       /* host-known max peaks of one spectrum (<= 65535)   */
+    float max_abs_mz;        /* host-known max |m/z| over all peaks               */
+} cs_spectra;
+
+/*
+ * CosineGreedy parameters (CudaCosineGreedy.py:24-56).  mz_power and
+ * intensity_power are folded into cs_spectra.mz_pow / int_pow.
+ */
+typedef struct cs_cosine_params {
+    double tolerance;        /* peaks match when |mz_q + shift - mz_r| <= tolerance
+                                (matchms find_matches bounds, inclusive)          */
+    double shift;            /* added to query m/z (matchms CosineGreedy: 0)      */
+    int64_t match_limit;     /* overflow flag = (#candidates >= match_limit);
+                                results stay exact beyond it                      */
+    int32_t light_candidates;/* candidates one lane settles in registers before the
+                                pair moves to the wave-cooperative path: one of
+                                1, 4, 8, 16; 0 = library default                  */
+    int32_t round_capacity;  /* candidates the cooperative path settles per round
+                                (power of two, 4..1024); 0 = library default      */
+    int32_t stages;          /* 0 = whole pipeline, else a mask of CS_STAGE_*:
+                                calls with PREPARE, then LIGHT, then HEAVY on one
+                                stream and one workspace equal one call with 0    */
+} cs_cosine_params;
+
+enum {
+    CS_STAGE_PREPARE = 1,    /* length-sort the queries into interleaved tiles    */
+    CS_STAGE_LIGHT = 2,      /* one lane per pair, <= light_candidates candidates */
+    CS_STAGE_HEAVY = 4       /* one wave per remaining pair                       */
+};
+
+/* ------------------------------------------------------------------------ */
+/* CosineGreedy all-pairs (dense output)                                      */
+/* ------------------------------------------------------------------------ */
+
+/* Bytes of device workspace cs_cosine_greedy needs for these shapes. */
+cs_status cs_cosine_greedy_workspace_size(const cs_spectra *refs, const cs_spectra *queries,
+                                          size_t *bytes);
+
+/*
+ * score[r*ld + q], matches[r*ld + q], overflow[r*ld + q] for every reference r
+ * and query q (row-major, ld >= queries->n_spectra).  Replaces the launcher at
+ * spectrum_similarity_functions.py:264-276 plus the per-tile loop of
+ * CudaCosineGreedy.matrix (CudaCosineGreedy.py:149-185): one call covers the
+ * whole R x Q matrix, asynchronously on `stream` (a hipStream_t, NULL = the
+ * null stream).  Semantics are matchms CosineGreedy.pair per pair:
+ *   score    float32 of the float64 greedy score,
+ *   matches  number of greedily accepted peak pairs (exact, never truncated),
+ *   overflow 1 iff the pair had >= match_limit candidate peak pairs.
+ */
+cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *queries,
+                           const cs_cosine_params *params, float *score, int32_t *matches,
+                           uint8_t *overflow, int64_t ld, void *workspace,
+                           size_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fingerprint similarity                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef enum cs_fingerprint_measure {
+    CS_JACCARD = 0,  /* I / (|a| + |b| - I)      vector_similarity_functions.py:42-46   */
+    CS_DICE = 1,     /* 2 * (I / (|a| + |b|))    vector_similarity_functions.py:87-90   */
+    CS_COSINE = 2    /* I / sqrt(|a| * |b|)      vector_similarity_functions.py:116-118 */
+} cs_fingerprint_measure;
+
+/* Bytes of device workspace cs_fingerprint_similarity needs. */
+cs_status cs_fingerprint_workspace_size(int64_t n_refs, int64_t n_queries, int64_t n_words,
+                                        size_t *bytes);
+
+/*
+ * out[r*ld + q] = measure(ref r, query q) as float32; 0/0 gives 0 (the
+ * reference's nan_to_num).  Fingerprints are bit-packed, n_words uint32 words
+ * per row (bit b of row x is word b/32, bit b%32).  Intersections and
+ * popcounts are exact int32 (int8 matrix-core products of 0/1 tiles).  Rows
+ * whose valid flag is 0 (missing fingerprint) get `empty_score` instead
+ * (CudaFingerprintSimilarity.py:147-154); a NULL valid array means all valid.
+ * If `intersections` is not NULL it receives the int32 counts [r*ld + q].
+ */
+cs_status cs_fingerprint_similarity(int measure, const uint32_t *ref_words, int64_t n_refs,
+                                    const uint32_t *query_words, int64_t n_queries,
+                                    int64_t n_words, const uint8_t *ref_valid,
+                                    const uint8_t *query_valid, float empty_score, float *out,
+                                    int32_t *intersections, int64_t ld, void *workspace,
+                                    size_t workspace_bytes, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COSINE_SIMILARITY_AMD_H */

@@ -1,0 +1,277 @@
+"""CosineGreedy on the GPU (libcsamd.so through the drop-in class and the
+device API) against the oracle.  Bar: matches / candidates-derived overflow
+bit-exact, score within 1e-5 absolute of the float64 oracle."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+SCORE_ATOL = 1e-5
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def spectra_list(p):
+    from cudams.spectrum import Spectrum
+
+    return [Spectrum(p.mz[p.offsets[i]:p.offsets[i + 1]],
+                     p.intensities[p.offsets[i]:p.offsets[i + 1]]) for i in range(p.n_spectra)]
+
+
+def assert_parity(res, want, match_limit=2048):
+    np.testing.assert_array_equal(res["matches"], want["matches"])
+    np.testing.assert_array_equal(res["overflow"], (want["n_candidates"] >= match_limit).astype(np.uint8))
+    np.testing.assert_allclose(res["score"], want["score"], rtol=0, atol=SCORE_ATOL)
+
+
+def device_run(refs, queries, tol=0.1, shift=0.0, mzp=0.0, inp=1.0, match_limit=2048, K=0, T=0,
+               ld_pad=0):
+    import torch
+
+    from cudams.device import cosine_greedy
+    from cudams.spectra import DeviceSpectra
+
+    R = DeviceSpectra(refs, mzp, inp)
+    Q = DeviceSpectra(queries, mzp, inp)
+    ld = Q.n_spectra + ld_pad
+    score = torch.full((R.n_spectra, ld), -7.0, dtype=torch.float32, device="cuda")
+    matches = torch.full((R.n_spectra, ld), -7, dtype=torch.int32, device="cuda")
+    overflow = torch.full((R.n_spectra, ld), 7, dtype=torch.uint8, device="cuda")
+    cosine_greedy(R, Q, tol, shift, match_limit, score, matches, overflow, light_candidates=K,
+                  round_capacity=T)
+    torch.cuda.synchronize()
+    out = {"score": score.cpu().numpy(), "matches": matches.cpu().numpy(),
+           "overflow": overflow.cpu().numpy()}
+    if ld_pad:
+        assert np.all(out["score"][:, Q.n_spectra:] == -7.0)
+        assert np.all(out["matches"][:, Q.n_spectra:] == -7)
+        out = {k: v[:, :Q.n_spectra] for k, v in out.items()}
+    return out
+
+
+def oracle_csr(oracle, refs, queries, **kw):
+    return oracle.cosine_greedy_matrix((refs.mz, refs.intensities, refs.offsets),
+                                       (queries.mz, queries.intensities, queries.offsets), **kw)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_known_answer_pairs(gpu, case):
+    """tests/similarity/test_cosine_greedy.py:27-120 of the reference, through pair()."""
+    from cudams.similarity import CudaCosineGreedy
+    from cudams.spectrum import Spectrum
+
+    with open(os.path.join(GOLDEN, "cosine_greedy_kats.json")) as fh:
+        k = json.load(fh)["pairs"][case]
+    say(f"KAT case {case}")
+    s1 = Spectrum(np.array(k["ref_mz"], float), np.array(k["ref_int"], float))
+    s2 = Spectrum(np.array(k["query_mz"], float), np.array(k["query_int"], float))
+    cg = CudaCosineGreedy(batch_size=8, tolerance=k["tolerance"], mz_power=k["mz_power"],
+                          intensity_power=k["intensity_power"])
+    score = cg.pair(s1, s2)
+    exp = k["expected_matches"]
+    mz1, i1 = np.array(k["ref_mz"], float), np.array(k["ref_int"], float)
+    mz2, i2 = np.array(k["query_mz"], float), np.array(k["query_int"], float)
+    p, q = k["mz_power"], k["intensity_power"]
+    num = ((mz1[exp[0]] ** p) * (i1[exp[0]] ** q) * (mz2[exp[1]] ** p) * (i2[exp[1]] ** q)).sum()
+    den = np.sqrt((((mz1 ** p) * (i1 ** q)) ** 2).sum()) * np.sqrt((((mz2 ** p) * (i2 ** q)) ** 2).sum())
+    assert score["score"] == pytest.approx(num / den, 0.0001)
+    assert score["matches"] == len(exp[0])
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_matrix_symmetry(gpu, symmetric):
+    from cudams.similarity import CudaCosineGreedy
+    from cudams.spectrum import Spectrum
+
+    s1 = Spectrum(np.array([100, 200, 300], float), np.array([0.1, 0.2, 1.0], float))
+    s2 = Spectrum(np.array([110, 190, 290], float), np.array([0.5, 0.2, 1.0], float))
+    spectra = [s1, s2]
+    scores = CudaCosineGreedy(batch_size=8).matrix(spectra, spectra, is_symmetric=symmetric)
+    assert scores.dtype.names == ("score", "matches", "overflow")
+    assert scores[0][0]["score"] == pytest.approx(scores[1][1]["score"], 0.000001)
+    assert scores[0][1]["score"] == pytest.approx(scores[1][0]["score"], 0.000001)
+
+
+def test_uniform_256_vs_oracle(gpu, oracle):
+    """The 256 x 256 uniform workload (refs seed 1, queries seed 2), tol 0.1."""
+    from cudams import synthetic
+
+    refs = synthetic.uniform_spectra(256, 1)
+    ques = synthetic.uniform_spectra(256, 2)
+    say("uniform 256x256 tol 0.1")
+    res = device_run(refs, ques, tol=0.1)
+    assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=0.1))
+
+
+def test_uniform_slice_fixture(gpu):
+    """Same workload, first 32 x 32, against the committed golden fixture."""
+    from cudams import synthetic
+
+    z = np.load(os.path.join(GOLDEN, "uniform_slice.npz"))
+    refs = synthetic.uniform_spectra(256, 1)
+    ques = synthetic.uniform_spectra(256, 2)
+    res = device_run(refs, ques, tol=0.1)
+    assert np.array_equal(refs.mz[: len(z["ref_mz"])], z["ref_mz"])
+    np.testing.assert_array_equal(res["matches"][:32, :32], z["matches"])
+    np.testing.assert_allclose(res["score"][:32, :32], z["score"], atol=SCORE_ATOL, rtol=0)
+
+
+def test_pesticides_matrix_and_golden(gpu):
+    """pesticides.mgf all-vs-all through CudaCosineGreedy.matrix, against the
+    committed oracle matrix and the matchms values of the reference notebook."""
+    from cudams.similarity import CudaCosineGreedy
+    from cudams.spectrum import load_from_mgf, normalize_intensities
+
+    spectra = [normalize_intensities(s) for s in load_from_mgf(os.path.join(GOLDEN, "pesticides.mgf"))]
+    say("pesticides 76x76")
+    res = CudaCosineGreedy(batch_size=512).matrix(spectra, spectra)
+    z = np.load(os.path.join(GOLDEN, "pesticides_oracle.npz"))
+    np.testing.assert_array_equal(res["matches"], z["matches"])
+    np.testing.assert_allclose(res["score"], z["score"], atol=SCORE_ATOL, rtol=0)
+    assert not res["overflow"].any()
+    with open(os.path.join(GOLDEN, "pesticides_matchms.json")) as fh:
+        g = json.load(fh)
+    q = g["query_index"]
+    col = [(res["score"][r, q], res["matches"][r, q], spectra[r].get("scans"))
+           for r in range(len(spectra)) if r != q]
+    col.sort(key=lambda t: -t[0])
+    for (s, m, scan), hit in zip(col, g["hits"]):
+        assert scan == hit["scan"] and m == hit["matches"]
+        assert abs(s - hit["score"]) < 5e-5
+
+
+@pytest.mark.parametrize("K,T", [(1, 4), (4, 8), (8, 512), (16, 1024)])
+def test_heavy_path_forced(gpu, oracle, K, T):
+    """Small light capacity and round capacity: most pairs go through the
+    wave-cooperative path, many over several rounds."""
+    from cudams import synthetic
+
+    refs = synthetic.lognormal_spectra(40, 21, cluster_fraction=0.5)
+    ques = synthetic.lognormal_spectra(70, 22, cluster_fraction=0.5)
+    for tol in (0.01, 0.5):
+        say(f"heavy K={K} T={T} tol={tol}")
+        res = device_run(refs, ques, tol=tol, K=K, T=T, match_limit=64)
+        assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=tol, match_limit=64), 64)
+
+
+def test_self_pairs_with_clusters(gpu, oracle):
+    """All-vs-all with dense clusters: self pairs carry thousands of candidates."""
+    from cudams import synthetic
+
+    p = synthetic.lognormal_spectra(96, 5)
+    say("clusters 96x96 tol 0.01")
+    res = device_run(p, p, tol=0.01)
+    want = oracle_csr(oracle, p, p, tolerance=0.01)
+    assert want["n_candidates"].max() > 1000
+    assert_parity(res, want)
+
+
+@pytest.mark.parametrize("mzp,inp,tol", [(0.5, 2.0, 1.0), (1.0, 0.5, 0.1), (0.0, 2.0, 0.02)])
+def test_powers(gpu, oracle, mzp, inp, tol):
+    from cudams import synthetic
+
+    refs = synthetic.uniform_spectra(64, 31)
+    ques = synthetic.uniform_spectra(80, 32)
+    say(f"powers {mzp} {inp} tol {tol}")
+    res = device_run(refs, ques, tol=tol, mzp=mzp, inp=inp)
+    assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=tol, mz_power=mzp,
+                                  intensity_power=inp))
+
+
+def test_float64_mz_and_shift(gpu, oracle):
+    """m/z that float32 cannot hold, many within ulps of the window edge, plus
+    a nonzero shift and a padded leading dimension."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(9)
+    mzs_r, its_r, mzs_q, its_q = [], [], [], []
+    for _ in range(50):
+        base = np.sort(rng.uniform(100, 900, rng.integers(1, 40)))
+        mzs_r.append(base)
+        its_r.append(rng.random(len(base)) + 0.01)
+        tol = 0.01
+        eps = rng.integers(-3, 4, len(base)) * np.spacing(base)
+        q = np.sort(base + rng.choice([-tol, tol], len(base)) + eps - 3.5)
+        mzs_q.append(q)
+        its_q.append(rng.random(len(q)) + 0.01)
+    refs = PackedSpectra.from_arrays(mzs_r, its_r)
+    ques = PackedSpectra.from_arrays(mzs_q, its_q)
+    say("float64 m/z + shift 3.5")
+    res = device_run(refs, ques, tol=0.01, shift=3.5, ld_pad=5)
+    assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=0.01, shift=3.5))
+
+
+def test_empty_and_ragged(gpu, oracle):
+    from cudams.spectra import PackedSpectra
+
+    mzs = [[], [100.0], [100.0, 100.05, 300.0], [], [99.95]]
+    its = [[], [1.0], [0.5, 1.0, 0.2], [], [0.7]]
+    p = PackedSpectra.from_arrays(mzs, its)
+    say("empty and ragged")
+    res = device_run(p, p, tol=0.1)
+    assert_parity(res, oracle_csr(oracle, p, p, tolerance=0.1))
+
+
+def test_overflow_flag_rule(gpu, oracle):
+    from cudams import synthetic
+
+    p = synthetic.lognormal_spectra(48, 8)
+    say("overflow rule, match_limit 16")
+    res = device_run(p, p, tol=0.01, match_limit=16)
+    want = oracle_csr(oracle, p, p, tolerance=0.01, match_limit=16)
+    assert want["overflow"].any()
+    assert_parity(res, want, 16)
+
+
+def test_sparse_output(gpu, oracle):
+    from cudams import synthetic
+    from cudams.similarity import CudaCosineGreedy
+
+    p = synthetic.uniform_spectra(120, 41)
+    spectra = spectra_list(p)
+    cg = CudaCosineGreedy(tolerance=0.1, batch_size=50, sparse_threshold=0.05)
+    say("sparse output")
+    sp = cg.matrix(spectra, spectra, array_type="sparse")
+    want = oracle_csr(oracle, p, p, tolerance=0.1)
+    keep = want["score"] >= 0.05
+    dense = sp.to_array("sparse_score")
+    assert set(zip(*np.nonzero(keep))) <= set(zip(sp.row.tolist(), sp.col.tolist())) | set(
+        zip(*np.nonzero(np.abs(want["score"] - 0.05) < SCORE_ATOL)))
+    np.testing.assert_allclose(dense[keep], want["score"][keep], atol=SCORE_ATOL, rtol=0)
+    np.testing.assert_array_equal(sp.to_array("sparse_matches")[keep], want["matches"][keep])
+
+
+def test_large_sampled_pairs(gpu, oracle):
+    """2048 x 2048 of the clustered lognormal workload: full-matrix invariants
+    plus 4000 random pairs against the oracle."""
+    from cudams import synthetic
+
+    p = synthetic.lognormal_spectra(2048, 5)
+    say("2048x2048 clustered tol 0.01")
+    res = device_run(p, p, tol=0.01)
+    n = np.diff(p.offsets)
+    assert np.all(np.diag(res["matches"]) == n)
+    np.testing.assert_allclose(np.diag(res["score"]), 1.0, atol=SCORE_ATOL)
+    assert np.all(res["matches"] >= 0)
+    assert np.all(res["matches"] <= np.minimum.outer(n, n))
+    assert np.all((res["score"] >= 0) & (res["score"] <= 1 + 1e-6))
+    rng = np.random.default_rng(0)
+    rr = rng.integers(0, 2048, 4000)
+    qq = rng.integers(0, 2048, 4000)
+    for r, q in zip(rr, qq):
+        a = slice(p.offsets[r], p.offsets[r + 1])
+        b = slice(p.offsets[q], p.offsets[q + 1])
+        s, m, c = oracle.cosine_greedy_pair(p.mz[a], p.intensities[a], p.mz[b], p.intensities[b],
+                                            tolerance=0.01)
+        assert res["matches"][r, q] == m
+        assert abs(res["score"][r, q] - s) <= SCORE_ATOL
+        assert res["overflow"][r, q] == (c >= 2048)
