@@ -518,14 +518,16 @@ Layout layout(void *base, const cs_spectra &R, const cs_spectra &Q) {
     // the shortest of tile t + 1: sum of tile maxima <= n_peaks / 64 + 2 * max_peaks.
     L.tile_elems = kWave * ((Q.n_peaks + kWave - 1) / kWave + 2 * (int64_t)Q.max_peaks +
                             L.ntiles + 1);
-    L.counters = c.take<unsigned long long>(4);
+    // query-only buffers first: their offsets do not depend on the references,
+    // so a PREPARE done once serves later calls with other references
     L.hist = c.take<int32_t>(L.nbins);
     L.cursor = c.take<int32_t>(L.nbins);
     L.order = c.take<int32_t>(Q.n_spectra);
     L.tile_len = c.take<int32_t>(L.ntiles);
     L.tile_base = c.take<int64_t>(L.ntiles + 1);
-    L.heavy_queue = c.take<unsigned long long>(L.heavy_cap > 0 ? L.heavy_cap : 1);
     L.tiles = c.take<float>(L.tile_elems);
+    L.counters = c.take<unsigned long long>(4);
+    L.heavy_queue = c.take<unsigned long long>(L.heavy_cap > 0 ? L.heavy_cap : 1);
     L.bytes = align_up(c.used, 256);
     return L;
 }
@@ -622,8 +624,6 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     if (stages & ~(CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY))
         return fail(CS_ERR_INVALID, "unknown stage bits 0x%x", stages);
     if (stages & CS_STAGE_PREPARE) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)4);
-    CS_LAUNCH_CHECK("k_fill_u64");
     hipLaunchKernelGGL(k_fill_i32, dim3(blocks(L.nbins, 256)), dim3(256), 0, hs, L.hist, L.nbins,
                        0);
     CS_LAUNCH_CHECK("k_fill_i32");
@@ -654,6 +654,9 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     const dim3 grid((unsigned)nrb, (unsigned)L.ntiles);
     const int64_t ml = params->match_limit;
     if (stages & CS_STAGE_LIGHT) {
+    // the heavy-pair queue of this call starts empty
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)4);
+    CS_LAUNCH_CHECK("k_fill_u64");
     switch (K) {
         case 1: st = launch_light<1>(grid, hs, R, nR, Q, nQ, L, tolc, tol, shift, ml, score, matches, overflow, ld); break;
         case 4: st = launch_light<4>(grid, hs, R, nR, Q, nQ, L, tolc, tol, shift, ml, score, matches, overflow, ld); break;

@@ -29,6 +29,8 @@ EXPORTED_SYMBOLS = (
     "cs_cosine_greedy",
     "cs_fingerprint_workspace_size",
     "cs_fingerprint_similarity",
+    "cs_topk_rows",
+    "cs_threshold_compact",
 )
 
 
@@ -103,6 +105,11 @@ def load(path: str = LIB_PATH):
     L.cs_fingerprint_similarity.restype = i32
     L.cs_fingerprint_similarity.argtypes = [i32, p, i64, p, i64, i64, p, p, ctypes.c_float, p,
                                             p, i64, p, sz, p]
+    L.cs_topk_rows.restype = i32
+    L.cs_topk_rows.argtypes = [p, p, i64, i64, i64, i32, p, p, p, p]
+    L.cs_threshold_compact.restype = i32
+    L.cs_threshold_compact.argtypes = [p, p, p, i64, i64, i64, ctypes.c_float, i64, p, p, i64,
+                                       p, p, p, p, p, p]
     _lib = L
     return L
 

@@ -113,3 +113,59 @@ def fingerprint_similarity(measure: str, ref_words: torch.Tensor, query_words: t
         query_words.data_ptr(), Q, W, ptr(ref_valid), ptr(query_valid),
         float(empty_score), out.data_ptr(), ptr(intersections), out.stride(0), buf.data_ptr(),
         buf.numel(), _stream_ptr(stream)))
+
+
+def topk_rows(score: torch.Tensor, k: int, matches: Optional[torch.Tensor] = None,
+              rows: Optional[int] = None, cols: Optional[int] = None, stream=None):
+    """Per row, the k best (score desc, column asc): (scores [rows, k] f32,
+    columns [rows, k] i32, matches [rows, k] i32 or None)."""
+    require_gpu()
+    L = _native.load()
+    rows = score.shape[0] if rows is None else rows
+    cols = score.shape[1] if cols is None else cols
+    if score.dtype != torch.float32 or score.stride(1) != 1:
+        raise ValueError("score must be a row-major float32 tensor")
+    if matches is not None and matches.stride(0) != score.stride(0):
+        raise ValueError("matches must share the row stride of score")
+    ts = torch.empty((rows, k), dtype=torch.float32, device=score.device)
+    ti = torch.empty((rows, k), dtype=torch.int32, device=score.device)
+    tm = None if matches is None else torch.empty((rows, k), dtype=torch.int32,
+                                                  device=score.device)
+    _native.check(L.cs_topk_rows(score.data_ptr(), None if matches is None else matches.data_ptr(),
+                                 rows, cols, score.stride(0), int(k), ts.data_ptr(), ti.data_ptr(),
+                                 None if tm is None else tm.data_ptr(), _stream_ptr(stream)))
+    return ts, ti, tm
+
+
+def threshold_compact(score: torch.Tensor, matches: torch.Tensor, overflow: torch.Tensor,
+                      threshold: float, row0: int = 0, rows: Optional[int] = None,
+                      cols: Optional[int] = None, stream=None):
+    """COO of entries with score >= threshold in row-major order, copied to the
+    host: (row i64, col i64, score f32, matches i32, overflow u8)."""
+    require_gpu()
+    L = _native.load()
+    rows = score.shape[0] if rows is None else rows
+    cols = score.shape[1] if cols is None else cols
+    dev = score.device
+    ld = score.stride(0)
+    offs = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    counts = torch.empty(max(rows, 1), dtype=torch.int32, device=dev)
+    sp = _stream_ptr(stream)
+    _native.check(L.cs_threshold_compact(score.data_ptr(), matches.data_ptr(), overflow.data_ptr(),
+                                         rows, cols, ld, float(threshold), int(row0),
+                                         offs.data_ptr(), counts.data_ptr(), 0, None, None, None,
+                                         None, None, sp))
+    total = int(offs[rows].item()) if rows else 0
+    n = max(total, 1)
+    r = torch.empty(n, dtype=torch.int64, device=dev)
+    c = torch.empty(n, dtype=torch.int64, device=dev)
+    s = torch.empty(n, dtype=torch.float32, device=dev)
+    m = torch.empty(n, dtype=torch.int32, device=dev)
+    o = torch.empty(n, dtype=torch.uint8, device=dev)
+    if total:
+        _native.check(L.cs_threshold_compact(score.data_ptr(), matches.data_ptr(),
+                                             overflow.data_ptr(), rows, cols, ld, float(threshold),
+                                             int(row0), offs.data_ptr(), counts.data_ptr(), total,
+                                             r.data_ptr(), c.data_ptr(), s.data_ptr(),
+                                             m.data_ptr(), o.data_ptr(), sp))
+    return tuple(x[:total].cpu().numpy() for x in (r, c, s, m, o))

@@ -89,7 +89,10 @@ typedef struct cs_cosine_params {
                                 (power of two, 4..1024); 0 = library default      */
     int32_t stages;          /* 0 = whole pipeline, else a mask of CS_STAGE_*:
                                 calls with PREPARE, then LIGHT, then HEAVY on one
-                                stream and one workspace equal one call with 0    */
+                                stream and one workspace equal one call with 0.
+                                PREPARE depends only on `queries`: once done, any
+                                number of LIGHT|HEAVY calls with other `refs` may
+                                reuse it (same workspace, same queries)           */
 } cs_cosine_params;
 
 enum {
@@ -151,6 +154,38 @@ cs_status cs_fingerprint_similarity(int measure, const uint32_t *ref_words, int6
                                     const uint8_t *query_valid, float empty_score, float *out,
                                     int32_t *intersections, int64_t ld, void *workspace,
                                     size_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Reductions over a dense score block                                        */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Per row, the k (1..16) best columns by score descending, ties to the lower
+ * column; NaN scores rank last.  top_score / top_index (/ top_matches if not
+ * NULL) are [rows, k] row-major; slots beyond `cols` get NaN / -1.  Replaces
+ * the host-side best-hits-per-query of matchms Scores.scores_by_query(sort=True)
+ * that the reference's users apply (tests/usecase/test_tutorial_pesticide.py:
+ * 38-41, 64-67).
+ */
+cs_status cs_topk_rows(const float *score, const int32_t *matches, int64_t rows, int64_t cols,
+                       int64_t ld, int k, float *top_score, int32_t *top_index,
+                       int32_t *top_matches, void *stream);
+
+/*
+ * Threshold compaction (CudaCosineGreedy.py:186-200): entries with
+ * score >= threshold, row-major order.  Two calls on one stream:
+ *   1. out_row == NULL: row_counts[rows] and row_offsets[rows + 1] (exclusive
+ *      prefix; row_offsets[rows] is the total) are written;
+ *   2. with outputs of `capacity` entries: (row0 + row, col, score[, matches]
+ *      [, overflow]) are written at row_offsets[row] onward (entries beyond
+ *      capacity are dropped).
+ */
+cs_status cs_threshold_compact(const float *score, const int32_t *matches,
+                               const uint8_t *overflow, int64_t rows, int64_t cols, int64_t ld,
+                               float threshold, int64_t row0, int64_t *row_offsets,
+                               int32_t *row_counts, int64_t capacity, int64_t *out_row,
+                               int64_t *out_col, float *out_score, int32_t *out_matches,
+                               uint8_t *out_overflow, void *stream);
 
 #ifdef __cplusplus
 }
