@@ -204,13 +204,37 @@ def bench_fingerprint(a) -> dict:
     return res
 
 
+def bench_copy(a) -> dict:
+    """A plain 1 GiB device-to-device copy: the calibration point for the
+    FETCH_SIZE / WRITE_SIZE counters (their reading for a copy of known size)."""
+    import torch
+
+    n = 1 << 28  # float32 elements = 1 GiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(a.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    return {"metric": "device copy GB/s (counter calibration)", "value": 2 * 4 * n / (ms / 1e3) / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": "copy_1GiB", "bytes_read": 4 * n,
+                                            "bytes_written": 4 * n},
+            "roofline": None, "cpu_baseline": None}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--workload", choices=["cosine", "fingerprint"], default="cosine")
+    ap.add_argument("--workload", choices=["cosine", "fingerprint", "copy"], default="cosine")
     ap.add_argument("--light-candidates", type=int, default=0)
     ap.add_argument("--round-capacity", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -225,7 +249,7 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
-    res = bench_cosine(a) if a.workload == "cosine" else bench_fingerprint(a)
+    res = {"cosine": bench_cosine, "fingerprint": bench_fingerprint, "copy": bench_copy}[a.workload](a)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -1,9 +1,9 @@
 // cg_lane.h -- per-pair CosineGreedy logic shared by the device kernels and
-// the host-side unit harness (tests/native/lane_harness.cpp).
+// the host-side unit harness (tests/native/tile_harness.cpp).
 //
 // Everything here is __host__ __device__ and free of wave intrinsics, so the
-// exact code a GPU lane runs for one (reference, query) pair can also be run
-// on the CPU against the oracle.
+// exact code a GPU lane runs to settle one (reference, query) pair can also be
+// run on the CPU against the oracle.
 //
 // Semantics (matchms CosineGreedy.pair as the reference calls it,
 // cudams/similarity/spectrum_similarity_functions.py:306-317):
@@ -95,67 +95,6 @@ struct LaneCandidates {
         matches = m;
     }
 };
-
-// Result of scanning one pair in a lane.
-struct LaneResult {
-    double score;     // un-normalised greedy sum
-    int matches;
-    int candidates;   // exact candidate count (capped at K + 1 when heavy)
-    bool heavy;       // more than K candidates: pair needs the cooperative path
-};
-
-// The lane scan: a merge over the reference (pointer lo .. k) and the query
-// (pointer j) with a float32 coarse window `tolc` that is wider than `tol` by
-// more than every rounding error involved, followed by the exact float64 test
-// on each coarse hit.
-//
-//   RefMz(k)       -> float   reference m/z (float32), k in [0, nr)
-//   QryMz(j)       -> float   query m/z + shift (float32), j in [0, nq];
-//                             must return +inf at j == nq
-//   Exact(k, j)    -> bool    float64 window test of (k, j)
-//   Value(k, j)    -> double  power product of (k, j)
-//
-// Each step advances exactly one of lo, j or k, so a pair costs at most
-// nr + nq + (#coarse hits) steps.
-template <int K, class RefMz, class QryMz, class Exact, class Value>
-CS_HD LaneResult scan_pair(int nr, int nq, float tolc, RefMz ref_mz, QryMz qry_mz, Exact exact,
-                           Value value, LaneCandidates<K> &cand) {
-    cand.reset();
-    LaneResult res{0.0, 0, 0, false};
-    if (nr <= 0 || nq <= 0) return res;
-    const float inf = __builtin_huge_valf();
-    int lo = 0, k = 0, j = 0, count = 0;
-    bool heavy = false;
-    float vk = ref_mz(0);
-    float vq = qry_mz(0);
-    bool active = true;
-    while (active) {
-        const float d = vk - vq;
-        const bool below = d < -tolc;  // reference peak under the query window (k == lo)
-        const bool above = d > tolc;   // reference peak over it (or k == nr)
-        if (!below && !above) {
-            if (exact(k, j)) {
-                if (count < K) {
-                    cand.insert(value(k, j), pack_ij(k, j));
-                } else {
-                    heavy = true;
-                }
-                ++count;
-            }
-        }
-        lo = below ? lo + 1 : lo;
-        k = (below || above) ? lo : k + 1;
-        j = above ? j + 1 : j;
-        const float t = ref_mz(k < nr ? k : nr - 1);
-        vk = k < nr ? t : inf;
-        vq = qry_mz(j);
-        active = (j < nq) && (lo < nr) && !heavy;
-    }
-    res.candidates = count;
-    res.heavy = heavy;
-    if (!heavy) cand.greedy(res.score, res.matches);
-    return res;
-}
 
 // Coarse float32 tolerance covering every rounding error of the float32 scan
 // relative to the float64 test: m/z rounding of both peaks, the shift and its

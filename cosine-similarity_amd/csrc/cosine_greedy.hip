@@ -25,14 +25,16 @@
 
 #include "cs_internal.h"
 #include "cg_lane.h"
+#include "cg_tile.h"
 
 namespace cs {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kLightThreads = 256;
-constexpr int kRefsPerBlock = 32;     // references one light workgroup walks
-constexpr int kTileLdsCap = 160;      // query peaks (incl. pad) per lane staged in LDS
+constexpr int kLightWaves = 8;
+constexpr int kLightThreads = kWave * kLightWaves;
+constexpr int kLightRefs = 64;        // references one light workgroup walks
+constexpr int kTileLdsCap = 5888;     // sorted tile peaks staged in LDS (larger: read from HBM)
 constexpr int kDefaultK = 8;
 constexpr int kDefaultRound = 512;
 constexpr int kHeavyGrid = 2048;
@@ -68,14 +70,8 @@ __device__ __forceinline__ double pair_value(const SpecView &R, int64_t pr, cons
 }
 
 // ---------------------------------------------------------------------------
-// Prep kernels
+// Prep kernels: query tiles and their peak index
 // ---------------------------------------------------------------------------
-
-__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t value) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = value;
-}
 
 __global__ void k_fill_u64(unsigned long long *p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -83,130 +79,186 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n) {
         p[i] = 0ull;
 }
 
-__device__ __forceinline__ int clamp_len(int64_t n, int maxbin) {
-    return n < 0 ? 0 : (n > maxbin ? maxbin : (int)n);
-}
-
-__global__ void k_len_hist(const int64_t *off, int64_t n, int maxbin, int32_t *hist) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
-         q += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&hist[clamp_len(off[q + 1] - off[q], maxbin)], 1);
-}
-
-// Exclusive scan of n int32 values by one 1024-thread block.
-// out64[i] = mult * sum(in[0..i)), out64[n] = mult * total; out32 likewise
-// (without the total).  Either output may be null.
-__global__ void __launch_bounds__(1024) k_scan_block(const int32_t *in, int64_t n, int64_t mult,
-                                                     int64_t *out64, int32_t *out32) {
-    __shared__ int64_t part[1024];
-    const int tid = threadIdx.x;
-    const int64_t chunk = (n + 1023) / 1024;
-    const int64_t b = tid * chunk;
-    const int64_t e = b + chunk < n ? b + chunk : n;
-    int64_t s = 0;
-    for (int64_t i = b; i < e; ++i) s += in[i];
-    part[tid] = s;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int64_t t = tid >= o ? part[tid - o] : 0;
-        __syncthreads();
-        part[tid] += t;
-        __syncthreads();
-    }
-    int64_t run = tid > 0 ? part[tid - 1] : 0;
-    for (int64_t i = b; i < e; ++i) {
-        const int32_t x = in[i];
-        if (out64) out64[i] = run * mult;
-        if (out32) out32[i] = (int32_t)run;
-        run += x;
-    }
-    if (tid == 1023 && out64) out64[n] = part[1023] * mult;
-}
-
-__global__ void k_len_scatter(const int64_t *off, int64_t n, int maxbin, int32_t *cursor,
-                              int32_t *order) {
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
-         q += (int64_t)gridDim.x * blockDim.x) {
-        const int pos = atomicAdd(&cursor[clamp_len(off[q + 1] - off[q], maxbin)], 1);
-        order[pos] = (int32_t)q;
-    }
-}
-
-// tile_len[t] = 1 + max peaks over the tile's queries (the +1 is the +inf pad row).
-__global__ void k_tile_len(const int64_t *off, const int32_t *order, int64_t nq, int64_t ntiles,
-                           int maxbin, int32_t *tile_len) {
+// Per tile: peak count, position of its sorted list, value range.
+__global__ void k_tile_meta(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles,
+                            float shift32, int32_t *tile_np, int64_t *tile_pbase, float *tile_lo,
+                            float *tile_iw) {
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
          t += (int64_t)gridDim.x * blockDim.x) {
-        int m = 0;
-        for (int l = 0; l < kWave; ++l) {
-            const int64_t qi = t * kWave + l;
-            if (qi < nq) {
-                const int q = order[qi];
-                const int len = clamp_len(off[q + 1] - off[q], maxbin);
-                m = len > m ? len : m;
+        const int64_t q0 = t * kTileQueries;
+        const int64_t q1 = q0 + kTileQueries < nq ? q0 + kTileQueries : nq;
+        float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
+        for (int64_t q = q0; q < q1; ++q) {
+            if (off[q + 1] > off[q]) {
+                const float a = mz[off[q]] + shift32, b = mz[off[q + 1] - 1] + shift32;
+                lo = a < lo ? a : lo;
+                hi = b > hi ? b : hi;
             }
         }
-        tile_len[t] = m + 1;
+        if (!(lo <= hi)) lo = hi = 0.0f;
+        tile_np[t] = (int32_t)(off[q1] - off[q0]);
+        tile_pbase[t] = off[q0] - off[0] + t;
+        tile_lo[t] = lo;
+        tile_iw[t] = tile_inv_width(lo, hi);
     }
 }
 
-__global__ void k_tile_build(const float *mz, const int64_t *off, const int32_t *order, int64_t nq,
-                             const int64_t *tile_base, const int32_t *tile_len, int64_t ntiles,
-                             float shift32, float *tiles) {
-    const float inf = __builtin_huge_valf();
+// Merge the tile's sorted per-query lists into one list sorted by
+// (value, lane, peak): each element's rank is its own index plus, for every
+// other list, the count of elements ordered before it (binary search).
+__global__ void __launch_bounds__(256)
+k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, float shift32,
+            const int32_t *tile_np, const int64_t *tile_pbase, float *smz, uint32_t *swho) {
+    __shared__ int64_t loff[kTileQueries + 1];
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int L = tile_len[t];
-        float *dst = tiles + tile_base[t];
-        for (int e = threadIdx.x; e < kWave * L; e += blockDim.x) {
-            const int k = e >> 6, lane = e & 63;
-            const int64_t qi = t * kWave + lane;
-            float v = inf;
-            if (qi < nq) {
-                const int q = order[qi];
-                const int64_t n = off[q + 1] - off[q];
-                if (k < n && k < L - 1) v = mz[off[q] + k] + shift32;
+        const int64_t q0 = t * kTileQueries;
+        const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
+        if ((int)threadIdx.x <= nl) loff[threadIdx.x] = off[q0 + threadIdx.x];
+        __syncthreads();
+        const int np = tile_np[t];
+        const int64_t pb = tile_pbase[t];
+        const int64_t g0 = loff[0];
+        for (int e = threadIdx.x; e < np; e += blockDim.x) {
+            const int64_t g = g0 + e;
+            int ql = 0;
+            while (ql + 1 < nl && loff[ql + 1] <= g) ++ql;
+            const int j = (int)(g - loff[ql]);
+            const float x = mz[g] + shift32;
+            int64_t rank = j;
+            for (int l = 0; l < nl; ++l) {
+                if (l == ql) continue;
+                int64_t a = loff[l], b = loff[l + 1];
+                const int64_t base = a;
+                if (l < ql) {  // count y <= x
+                    while (a < b) {
+                        const int64_t m = (a + b) >> 1;
+                        if (mz[m] + shift32 <= x) a = m + 1;
+                        else b = m;
+                    }
+                } else {       // count y < x
+                    while (a < b) {
+                        const int64_t m = (a + b) >> 1;
+                        if (mz[m] + shift32 < x) a = m + 1;
+                        else b = m;
+                    }
+                }
+                rank += a - base;
             }
-            dst[e] = v;
+            smz[pb + rank] = x;
+            swho[pb + rank] = tile_who(ql, j);
+        }
+        if (threadIdx.x == 0) {
+            smz[pb + np] = __builtin_huge_valf();
+            swho[pb + np] = 0u;
+        }
+        __syncthreads();
+    }
+}
+
+// dir[b] = first list position whose bucket is >= b, for b in [0, kTileBuckets].
+__global__ void __launch_bounds__(256)
+k_tile_dir(int64_t ntiles, const int32_t *tile_np, const int64_t *tile_pbase,
+           const float *tile_lo, const float *tile_iw, const float *smz, int32_t *dir) {
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int np = tile_np[t];
+        const float *s = smz + tile_pbase[t];
+        const float lo = tile_lo[t], iw = tile_iw[t];
+        int32_t *d = dir + t * (kTileBuckets + 1);
+        for (int p = threadIdx.x; p <= np; p += blockDim.x) {
+            const int bp = p < np ? tile_bucket(s[p], lo, iw) : kTileBuckets;
+            const int bq = p > 0 ? tile_bucket(s[p - 1], lo, iw) : -1;
+            for (int b = bq + 1; b <= bp; ++b) d[b] = p;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Light kernel: one lane per pair
+// Light kernel: one wave per (reference, tile), lanes over reference peaks
 // ---------------------------------------------------------------------------
+
+struct LightArgs {
+    SpecView R, Q;
+    int64_t nR, nQ;
+    const int32_t *tile_np;
+    const int64_t *tile_pbase;
+    const float *tile_lo, *tile_iw;
+    const float *smz;
+    const uint32_t *swho;
+    const int32_t *dir;
+    int lds_cap;         // sorted peaks a tile may hold in LDS
+    float tolc;
+    double tol, shift;
+    int exact_from_tile; // query m/z in the sorted list are the exact values
+    int64_t match_limit;
+    float *score;
+    int32_t *matches;
+    uint8_t *overflow;
+    int64_t ld;
+    unsigned long long *heavy_queue;
+    int64_t heavy_cap;
+    unsigned long long *counters;
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Bytes of dynamic LDS of k_light<K> for a tile capacity `cap`.
+constexpr size_t light_lds_bytes(int K, int cap) {
+    return (size_t)(cap + 1) * 8                                  // smz + swho
+           + ((size_t)(kTileBuckets + 1) * 2 + 15) / 16 * 16      // dir (u16)
+           + (size_t)kTileQueries * (8 + 8)                       // qoff, qnorm
+           + (size_t)kLightWaves * kTileQueries * (4 + 12 * K);   // cnt + slots
+}
 
 template <int K>
-__global__ void __launch_bounds__(kLightThreads)
-k_light(SpecView R, int64_t nR, SpecView Q, int64_t nQ, const int32_t *__restrict__ order,
-        const int64_t *__restrict__ tile_base, const int32_t *__restrict__ tile_len,
-        const float *__restrict__ tiles, float tolc, double tol, double shift,
-        int64_t match_limit, float *__restrict__ score, int32_t *__restrict__ matches,
-        uint8_t *__restrict__ overflow, int64_t ld, unsigned long long *__restrict__ heavy_queue,
-        int64_t heavy_cap, unsigned long long *__restrict__ counters) {
-    extern __shared__ float lds_tile[];
+__global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
+    extern __shared__ __align__(16) unsigned char light_lds[];
     __shared__ int next_ref;
+    const int cap = a.lds_cap;
+    // LDS layout (see light_lds_bytes)
+    int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
+    double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
+    double *sv_all = qnorm_l + kTileQueries;
+    uint32_t *sij_all = reinterpret_cast<uint32_t *>(sv_all + kLightWaves * kTileQueries * K);
+    int *cnt_all = reinterpret_cast<int *>(sij_all + kLightWaves * kTileQueries * K);
+    float *smz_l = reinterpret_cast<float *>(cnt_all + kLightWaves * kTileQueries);
+    uint32_t *swho_l = reinterpret_cast<uint32_t *>(smz_l + cap + 1);
+    uint16_t *dir_l = reinterpret_cast<uint16_t *>(swho_l + cap + 1);
 
-    const int64_t rb = blockIdx.x;
-    const int64_t t = blockIdx.y;
-    const int L = tile_len[t];
-    const float *gt = tiles + tile_base[t];
-    const bool in_lds = L <= kTileLdsCap;
-    if (in_lds)
-        for (int e = threadIdx.x; e < kWave * L; e += kLightThreads) lds_tile[e] = gt[e];
-    if (threadIdx.x == 0) next_ref = 0;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t rb = blockIdx.x, t = blockIdx.y;
+    const int np = a.tile_np[t];
+    const int64_t pb = a.tile_pbase[t];
+    const float lo = a.tile_lo[t], iw = a.tile_iw[t];
+    const float *gsmz = a.smz + pb;
+    const uint32_t *gswho = a.swho + pb;
+    const int32_t *gdir = a.dir + t * (kTileBuckets + 1);
+    const bool in_lds = np <= cap;
+    const int64_t q0 = t * kTileQueries;
+    if (in_lds) {
+        for (int e = tid; e <= np; e += kLightThreads) {
+            smz_l[e] = gsmz[e];
+            swho_l[e] = gswho[e];
+        }
+        for (int b = tid; b <= kTileBuckets; b += kLightThreads) dir_l[b] = (uint16_t)gdir[b];
+    }
+    if (tid < kTileQueries) {
+        const int64_t q = q0 + tid < a.nQ ? q0 + tid : q0;
+        qoff_l[tid] = a.Q.off[q];
+        qnorm_l[tid] = a.Q.norm[q];
+    }
+    if (tid == 0) next_ref = 0;
+    int *cnt = cnt_all + wave * kTileQueries;
+    double *sv = sv_all + wave * kTileQueries * K;
+    uint32_t *sij = sij_all + wave * kTileQueries * K;
+    if (lane < kTileQueries) cnt[lane] = 0;
     __syncthreads();
 
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t qi = t * kWave + lane;
-    const bool valid = qi < nQ;
-    const int q = valid ? order[qi] : 0;
-    const int64_t qoff = Q.off[q];
-    int nq = valid ? (int)(Q.off[q + 1] - qoff) : 0;
-    nq = nq < L - 1 ? nq : L - 1;  // the tile holds L - 1 peaks + the pad row
-    const double qnorm = Q.norm[q];
-    const int64_t r0 = rb * kRefsPerBlock;
-    const int64_t r1 = r0 + kRefsPerBlock < nR ? r0 + kRefsPerBlock : nR;
-
+    const int64_t r0 = rb * kLightRefs;
+    const int64_t r1 = r0 + kLightRefs < a.nR ? r0 + kLightRefs : a.nR;
+    const SpecView &R = a.R, &Q = a.Q;
     for (;;) {
         int rr = 0;
         if (lane == 0) rr = atomicAdd(&next_ref, 1);
@@ -215,37 +267,66 @@ k_light(SpecView R, int64_t nR, SpecView Q, int64_t nQ, const int32_t *__restric
         if (r >= r1) break;
         const int64_t roff = R.off[r];
         const int nr = (int)(R.off[r + 1] - roff);
-        const float *rmz = R.mz + roff;
-
-        LaneCandidates<K> cand;
-        auto ref_mz = [&](int k) { return rmz[k]; };
-        auto exact = [&](int k, int j) {
-            return exact_window(exact_mz(R, roff + k), exact_mz(Q, qoff + j), tol, shift);
-        };
-        auto value = [&](int k, int j) { return pair_value(R, roff + k, Q, qoff + j); };
-        LaneResult res;
-        if (in_lds) {
-            auto qry = [&](int j) { return lds_tile[j * kWave + lane]; };
-            res = scan_pair<K>(nr, nq, tolc, ref_mz, qry, exact, value, cand);
-        } else {
-            auto qry = [&](int j) { return gt[(int64_t)j * kWave + lane]; };
-            res = scan_pair<K>(nr, nq, tolc, ref_mz, qry, exact, value, cand);
-        }
-        if (valid) {
-            const int64_t o = r * ld + q;
-            if (res.heavy) {
-                const unsigned long long slot = atomicAdd(&counters[0], 1ull);
-                if ((int64_t)slot < heavy_cap)
-                    heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
-                score[o] = 0.0f;
-                matches[o] = -1;
-                overflow[o] = 0;
-            } else {
-                score[o] = res.candidates ? (float)(res.score / (R.norm[r] * qnorm)) : 0.0f;
-                matches[o] = res.matches;
-                overflow[o] = (uint8_t)(res.candidates >= match_limit);
+        for (int ib = 0; ib < nr; ib += kWave) {
+            const int i = ib + lane;
+            if (i < nr) {
+                const float ri = R.mz[roff + i];
+                const float x0 = ri - a.tolc, x1 = ri + a.tolc;
+                auto emit = [&](uint32_t who, float qv) {
+                    const int ql = (int)(who >> 16), j = (int)(who & 0xffffu);
+                    const int64_t qp = qoff_l[ql] + j;
+                    const double mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qp);
+                    const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+                    if (exact_window(mr, mq, a.tol, a.shift)) {
+                        const double v = pair_value(R, roff + i, Q, qp);
+                        const int pos = atomicAdd(&cnt[ql], 1);
+                        if (pos < K) {
+                            sv[ql * K + pos] = v;
+                            sij[ql * K + pos] = pack_ij(i, j);
+                        }
+                    }
+                };
+                if (in_lds) {
+                    tile_probe(x0, x1, lo, iw, [&](int p) { return smz_l[p]; },
+                               [&](int b) { return (int)dir_l[b]; },
+                               [&](int p) { emit(swho_l[p], smz_l[p]); });
+                } else {
+                    tile_probe(x0, x1, lo, iw, [&](int p) { return gsmz[p]; },
+                               [&](int b) { return (int)gdir[b]; },
+                               [&](int p) { emit(gswho[p], gsmz[p]); });
+                }
             }
         }
+        wave_sync();
+        if (lane < kTileQueries) {
+            const int n = cnt[lane];
+            cnt[lane] = 0;
+            const int64_t q = q0 + lane;
+            if (q < a.nQ) {
+                const int64_t o = r * a.ld + q;
+                if (n > K) {
+                    const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
+                    if ((int64_t)slot < a.heavy_cap)
+                        a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
+                    a.score[o] = 0.0f;
+                    a.matches[o] = -1;
+                    a.overflow[o] = 0;
+                } else {
+                    LaneCandidates<K> c;
+                    c.reset();
+#pragma unroll
+                    for (int s = 0; s < K; ++s)
+                        if (s < n) c.insert(sv[lane * K + s], sij[lane * K + s]);
+                    double sc = 0.0;
+                    int m = 0;
+                    c.greedy(sc, m);
+                    a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[lane])) : 0.0f;
+                    a.matches[o] = m;
+                    a.overflow[o] = (uint8_t)(n >= a.match_limit);
+                }
+            }
+        }
+        wave_sync();
     }
 }
 
@@ -498,36 +579,37 @@ __global__ void __launch_bounds__(kWave) k_heavy(HeavyArgs a) {
 // ---------------------------------------------------------------------------
 
 struct Layout {
-    unsigned long long *counters;
-    int32_t *hist, *cursor, *order, *tile_len;
-    int64_t *tile_base;
-    unsigned long long *heavy_queue;
-    float *tiles;
-    int64_t ntiles, nbins, heavy_cap, tile_elems;
+    int32_t *tile_np;
+    int64_t *tile_pbase;
+    float *tile_lo, *tile_iw, *smz;
+    uint32_t *swho;
+    int32_t *dir;
+    unsigned long long *counters, *heavy_queue;
+    int64_t ntiles, heavy_cap;
     size_t bytes;
 };
 
 Layout layout(void *base, const cs_spectra &R, const cs_spectra &Q) {
     Layout L{};
     Carver c(base);
-    L.ntiles = (Q.n_spectra + kWave - 1) / kWave;
-    L.nbins = (int64_t)Q.max_peaks + 1;
-    L.heavy_cap = R.n_spectra * Q.n_spectra < kHeavyQueueCap ? R.n_spectra * Q.n_spectra
-                                                             : kHeavyQueueCap;
-    // Tiles are length-sorted, so the longest query of tile t is no longer than
-    // the shortest of tile t + 1: sum of tile maxima <= n_peaks / 64 + 2 * max_peaks.
-    L.tile_elems = kWave * ((Q.n_peaks + kWave - 1) / kWave + 2 * (int64_t)Q.max_peaks +
-                            L.ntiles + 1);
+    L.ntiles = (Q.n_spectra + kTileQueries - 1) / kTileQueries;
+    // fixed capacity: the workspace size depends on the queries only, so one
+    // prepared workspace serves any reference set (pairs beyond the capacity
+    // are found by scanning for the -1 sentinel instead)
+    (void)R;
+    L.heavy_cap = kHeavyQueueCap;
+    const int64_t T = L.ntiles > 0 ? L.ntiles : 1;
     // query-only buffers first: their offsets do not depend on the references,
     // so a PREPARE done once serves later calls with other references
-    L.hist = c.take<int32_t>(L.nbins);
-    L.cursor = c.take<int32_t>(L.nbins);
-    L.order = c.take<int32_t>(Q.n_spectra);
-    L.tile_len = c.take<int32_t>(L.ntiles);
-    L.tile_base = c.take<int64_t>(L.ntiles + 1);
-    L.tiles = c.take<float>(L.tile_elems);
+    L.tile_np = c.take<int32_t>(T);
+    L.tile_pbase = c.take<int64_t>(T);
+    L.tile_lo = c.take<float>(T);
+    L.tile_iw = c.take<float>(T);
+    L.smz = c.take<float>(Q.n_peaks + T);
+    L.swho = c.take<uint32_t>(Q.n_peaks + T);
+    L.dir = c.take<int32_t>(T * (kTileBuckets + 1));
     L.counters = c.take<unsigned long long>(4);
-    L.heavy_queue = c.take<unsigned long long>(L.heavy_cap > 0 ? L.heavy_cap : 1);
+    L.heavy_queue = c.take<unsigned long long>(L.heavy_cap);
     L.bytes = align_up(c.used, 256);
     return L;
 }
@@ -550,14 +632,38 @@ cs_status check_spectra(const cs_spectra *s, const char *name) {
     return CS_OK;
 }
 
+// Largest tile capacity whose k_light<K> LDS fits the device limit (cached).
+int light_capacity(int K) {
+    static int max_lds = -1;
+    if (max_lds < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 65536;
+        max_lds = v;
+    }
+    const size_t fixed = light_lds_bytes(K, 0) + 64;  // + static next_ref and slack
+    const size_t room = (size_t)max_lds > fixed ? (size_t)max_lds - fixed : 0;
+    int cap = (int)(room / 8) - 1;
+    // two workgroups per CU when the CU has 160 KiB
+    const size_t two = (size_t)80 * 1024 > fixed ? (size_t)80 * 1024 - fixed : 0;
+    const int cap2 = (int)(two / 8) - 1;
+    cap = cap < cap2 ? cap : cap2;
+    return cap < kTileLdsCap ? (cap > 0 ? cap : 0) : kTileLdsCap;
+}
+
 template <int K>
-cs_status launch_light(dim3 grid, hipStream_t st, const SpecView &R, int64_t nR, const SpecView &Q,
-                       int64_t nQ, const Layout &L, float tolc, double tol, double shift,
-                       int64_t ml, float *score, int32_t *matches, uint8_t *overflow, int64_t ld) {
-    const size_t lds = sizeof(float) * kWave * kTileLdsCap;
-    hipLaunchKernelGGL(k_light<K>, grid, dim3(kLightThreads), lds, st, R, nR, Q, nQ, L.order,
-                       L.tile_base, L.tile_len, L.tiles, tolc, tol, shift, ml, score, matches,
-                       overflow, ld, L.heavy_queue, L.heavy_cap, L.counters);
+cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
+    a.lds_cap = light_capacity(K);
+    const size_t lds = light_lds_bytes(K, a.lds_cap);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_light<K>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(k_light<K>, grid, dim3(kLightThreads), lds, st, a);
     CS_LAUNCH_CHECK("k_light");
     return CS_OK;
 }
@@ -613,7 +719,6 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     const float mzmax = Rs.max_abs_mz > Qs.max_abs_mz ? Rs.max_abs_mz : Qs.max_abs_mz;
     const float tolc = coarse_tolerance(tol, shift, mzmax);
     const float shift32 = (float)shift;
-    const int maxbin = Qs.max_peaks;
     auto blocks = [](int64_t n, int per) {
         int64_t b = (n + per - 1) / per;
         return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
@@ -624,46 +729,58 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     if (stages & ~(CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY))
         return fail(CS_ERR_INVALID, "unknown stage bits 0x%x", stages);
     if (stages & CS_STAGE_PREPARE) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(L.nbins, 256)), dim3(256), 0, hs, L.hist, L.nbins,
-                       0);
-    CS_LAUNCH_CHECK("k_fill_i32");
-    hipLaunchKernelGGL(k_len_hist, dim3(blocks(nQ, 256)), dim3(256), 0, hs, Q.off, nQ, maxbin,
-                       L.hist);
-    CS_LAUNCH_CHECK("k_len_hist");
-    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(1024), 0, hs, L.hist, L.nbins, (int64_t)1,
-                       (int64_t *)nullptr, L.cursor);
-    CS_LAUNCH_CHECK("k_scan_block");
-    hipLaunchKernelGGL(k_len_scatter, dim3(blocks(nQ, 256)), dim3(256), 0, hs, Q.off, nQ, maxbin,
-                       L.cursor, L.order);
-    CS_LAUNCH_CHECK("k_len_scatter");
-    hipLaunchKernelGGL(k_tile_len, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, Q.off, L.order,
-                       nQ, L.ntiles, maxbin, L.tile_len);
-    CS_LAUNCH_CHECK("k_tile_len");
-    hipLaunchKernelGGL(k_scan_block, dim3(1), dim3(1024), 0, hs, L.tile_len, L.ntiles,
-                       (int64_t)kWave, L.tile_base, (int32_t *)nullptr);
-    CS_LAUNCH_CHECK("k_scan_block");
-    hipLaunchKernelGGL(k_tile_build, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.mz, Q.off,
-                       L.order, nQ, L.tile_base, L.tile_len, L.ntiles, shift32, L.tiles);
-    CS_LAUNCH_CHECK("k_tile_build");
+        hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, Q.mz, Q.off,
+                           nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw);
+        CS_LAUNCH_CHECK("k_tile_meta");
+        hipLaunchKernelGGL(k_tile_sort, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.mz, Q.off,
+                           nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.smz, L.swho);
+        CS_LAUNCH_CHECK("k_tile_sort");
+        hipLaunchKernelGGL(k_tile_dir, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, L.ntiles,
+                           L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz, L.dir);
+        CS_LAUNCH_CHECK("k_tile_dir");
     }
 
-    const int64_t nrb = (nR + kRefsPerBlock - 1) / kRefsPerBlock;
+    const int64_t nrb = (nR + kLightRefs - 1) / kLightRefs;
     if (nrb > INT_MAX || L.ntiles > 65535)
         return fail(CS_ERR_UNSUPPORTED, "grid too large (%lld ref blocks, %lld query tiles)",
                     (long long)nrb, (long long)L.ntiles);
-    const dim3 grid((unsigned)nrb, (unsigned)L.ntiles);
     const int64_t ml = params->match_limit;
     if (stages & CS_STAGE_LIGHT) {
-    // the heavy-pair queue of this call starts empty
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)4);
-    CS_LAUNCH_CHECK("k_fill_u64");
-    switch (K) {
-        case 1: st = launch_light<1>(grid, hs, R, nR, Q, nQ, L, tolc, tol, shift, ml, score, matches, overflow, ld); break;
-        case 4: st = launch_light<4>(grid, hs, R, nR, Q, nQ, L, tolc, tol, shift, ml, score, matches, overflow, ld); break;
-        case 16: st = launch_light<16>(grid, hs, R, nR, Q, nQ, L, tolc, tol, shift, ml, score, matches, overflow, ld); break;
-        default: st = launch_light<8>(grid, hs, R, nR, Q, nQ, L, tolc, tol, shift, ml, score, matches, overflow, ld); break;
-    }
-    if (st != CS_OK) return st;
+        // the heavy-pair queue of this call starts empty
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)4);
+        CS_LAUNCH_CHECK("k_fill_u64");
+        LightArgs la{};
+        la.R = R;
+        la.Q = Q;
+        la.nR = nR;
+        la.nQ = nQ;
+        la.tile_np = L.tile_np;
+        la.tile_pbase = L.tile_pbase;
+        la.tile_lo = L.tile_lo;
+        la.tile_iw = L.tile_iw;
+        la.smz = L.smz;
+        la.swho = L.swho;
+        la.dir = L.dir;
+        la.tolc = tolc;
+        la.tol = tol;
+        la.shift = shift;
+        la.exact_from_tile = (Qs.mz_exact == nullptr && shift == 0.0) ? 1 : 0;
+        la.match_limit = ml;
+        la.score = score;
+        la.matches = matches;
+        la.overflow = overflow;
+        la.ld = ld;
+        la.heavy_queue = L.heavy_queue;
+        la.heavy_cap = L.heavy_cap;
+        la.counters = L.counters;
+        const dim3 grid((unsigned)nrb, (unsigned)L.ntiles);
+        switch (K) {
+            case 1: st = launch_light<1>(grid, hs, la); break;
+            case 4: st = launch_light<4>(grid, hs, la); break;
+            case 16: st = launch_light<16>(grid, hs, la); break;
+            default: st = launch_light<8>(grid, hs, la); break;
+        }
+        if (st != CS_OK) return st;
     }
     if (!(stages & CS_STAGE_HEAVY)) return CS_OK;
 

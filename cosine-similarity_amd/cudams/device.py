@@ -22,9 +22,15 @@ class Workspace:
         self.device = device
         self.buf: Optional[torch.Tensor] = None
 
-    def get(self, nbytes: int) -> torch.Tensor:
+    def get(self, nbytes: int, keep: bool = False) -> torch.Tensor:
+        """The buffer, grown to ``nbytes`` if needed.  ``keep=True`` is for calls
+        that read state an earlier call left in the buffer: growing would lose
+        it, so it raises instead."""
         nbytes = max(int(nbytes), 256)
         if self.buf is None or self.buf.numel() < nbytes:
+            if keep:
+                raise ValueError("workspace too small for a call that reuses its prepared "
+                                 "state; run the PREPARE stage on it first")
             self.buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         return self.buf
 
@@ -65,7 +71,10 @@ def cosine_greedy(refs: DeviceSpectra, queries: DeviceSpectra, tolerance: float,
             raise ValueError("outputs must be row-major with one common row stride")
     ws = workspace or Workspace(score.device)
     nbytes = cosine_greedy_workspace_bytes(refs, queries)
-    buf = ws.get(nbytes)
+    reuse = stages != 0 and not (stages & _native.CS_STAGE_PREPARE)
+    if reuse and workspace is None:
+        raise ValueError("stages without PREPARE need the workspace a PREPARE call filled")
+    buf = ws.get(nbytes, keep=reuse)
     rs, qs = refs.struct(), queries.struct()
     p = _native.CsCosineParams(float(tolerance), float(shift), int(match_limit),
                                int(light_candidates), int(round_capacity), int(stages))

@@ -105,7 +105,8 @@ enum {
 /* CosineGreedy all-pairs (dense output)                                      */
 /* ------------------------------------------------------------------------ */
 
-/* Bytes of device workspace cs_cosine_greedy needs for these shapes. */
+/* Bytes of device workspace cs_cosine_greedy needs for these shapes.  The
+ * size depends on `queries` only (refs is accepted for symmetry). */
 cs_status cs_cosine_greedy_workspace_size(const cs_spectra *refs, const cs_spectra *queries,
                                           size_t *bytes);
 

@@ -33,24 +33,23 @@ def oracle():
     return o
 
 
-def build_lane_harness() -> str:
-    so = os.path.join(NATIVE, "liblane_harness.so")
-    src = os.path.join(NATIVE, "lane_harness.cpp")
-    hdr = os.path.join(PKG, "csrc", "cg_lane.h")
-    if (not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src)
-            or os.path.getmtime(so) < os.path.getmtime(hdr)):
+def build_tile_harness() -> str:
+    so = os.path.join(NATIVE, "libtile_harness.so")
+    src = os.path.join(NATIVE, "tile_harness.cpp")
+    deps = [src] + [os.path.join(PKG, "csrc", h) for h in ("cg_lane.h", "cg_tile.h")]
+    if not os.path.exists(so) or any(os.path.getmtime(so) < os.path.getmtime(d) for d in deps):
         subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                         f"-I{os.path.join(PKG, 'csrc')}", "-o", so, src], check=True)
     return so
 
 
 @pytest.fixture(scope="session")
-def lane_harness():
+def tile_harness():
     import ctypes
 
-    L = ctypes.CDLL(build_lane_harness())
+    L = ctypes.CDLL(build_tile_harness())
     d, p, i = ctypes.c_double, ctypes.c_void_p, ctypes.c_int
-    L.lane_scan.argtypes = [i, p, p, i, p, p, i, d, d, d, d, d, p, p, p, p]
+    L.tile_scan.argtypes = [i, p, p, p, i, p, p, p, i, d, d, d, d, d, p, p, p, p]
     L.coarse_tolerance.argtypes = [d, d, d]
     L.coarse_tolerance.restype = ctypes.c_float
     return L

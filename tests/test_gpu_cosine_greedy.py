@@ -275,3 +275,50 @@ def test_large_sampled_pairs(gpu, oracle):
         assert res["matches"][r, q] == m
         assert abs(res["score"][r, q] - s) <= SCORE_ATOL
         assert res["overflow"][r, q] == (c >= 2048)
+
+
+def test_tiles_larger_than_lds(gpu, oracle):
+    """Query tiles whose sorted peak list exceeds the LDS capacity take the
+    HBM-resident index path; tiles of both kinds in one call."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(12)
+    mzs, its = [], []
+    for x in range(96):
+        k = int(rng.integers(250, 400)) if x < 40 else int(rng.integers(5, 60))
+        mzs.append(np.sort(rng.uniform(50, 1000, k).astype(np.float32)).astype(np.float64))
+        its.append(rng.random(k) + 0.01)
+    ques = PackedSpectra.from_arrays(mzs, its)
+    refs = PackedSpectra.from_arrays(mzs[::3], its[::3])
+    say("tiles beyond LDS, tol 0.05")
+    res = device_run(refs, ques, tol=0.05)
+    assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=0.05))
+
+
+def test_stage_reuse_with_other_references(gpu, oracle):
+    """PREPARE once for a query set, then LIGHT|HEAVY with two different
+    reference sets on the same workspace."""
+    import torch
+
+    from cudams import _native, synthetic
+    from cudams.device import Workspace, cosine_greedy
+    from cudams.spectra import DeviceSpectra
+
+    lib = synthetic.lognormal_spectra(300, 61)
+    Q = DeviceSpectra(lib)
+    ws = Workspace("cuda")
+    first = True
+    for seed, n in ((62, 17), (63, 130)):
+        refs = synthetic.lognormal_spectra(n, seed)
+        R = DeviceSpectra(refs)
+        s = torch.empty((n, 300), dtype=torch.float32, device="cuda")
+        m = torch.empty((n, 300), dtype=torch.int32, device="cuda")
+        o = torch.empty((n, 300), dtype=torch.uint8, device="cuda")
+        if first:
+            cosine_greedy(R, Q, 0.01, 0.0, 2048, s, m, o, ws, stages=_native.CS_STAGE_PREPARE)
+            first = False
+        cosine_greedy(R, Q, 0.01, 0.0, 2048, s, m, o, ws,
+                      stages=_native.CS_STAGE_LIGHT | _native.CS_STAGE_HEAVY)
+        torch.cuda.synchronize()
+        res = {"score": s.cpu().numpy(), "matches": m.cpu().numpy(), "overflow": o.cpu().numpy()}
+        assert_parity(res, oracle_csr(oracle, refs, lib, tolerance=0.01))

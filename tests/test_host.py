@@ -176,3 +176,34 @@ def test_argbatch():
     from cudams.utils import argbatch
 
     assert list(argbatch(list(range(5)), 2)) == [(0, 2), (2, 4), (4, 6)]
+
+
+def test_workspace_size_depends_on_queries_only():
+    """A prepared workspace must serve any later reference set: its size may
+    not depend on the references (growing it would drop the prepared tiles)."""
+    from cudams import _native
+
+    L = _native.load()
+
+    def ws(R, Q, peaks_q):
+        r = _native.CsSpectra(R, R * 95, 8, 8, None, None, 8, 8, 1024, 1000.0)
+        q = _native.CsSpectra(Q, peaks_q, 8, 8, None, None, 8, 8, 1024, 1000.0)
+        n = ctypes.c_size_t()
+        _native.check(L.cs_cosine_greedy_workspace_size(ctypes.byref(r), ctypes.byref(q),
+                                                        ctypes.byref(n)))
+        return n.value
+
+    assert ws(17, 300, 28500) == ws(130, 300, 28500) == ws(500000, 300, 28500)
+    assert ws(17, 600, 57000) > ws(17, 300, 28500)
+
+
+def test_workspace_reuse_never_reallocates():
+    import torch
+
+    from cudams.device import Workspace
+
+    w = Workspace("cpu")
+    b = w.get(1000)
+    assert w.get(500, keep=True) is b
+    with pytest.raises(ValueError):
+        w.get(5000, keep=True)
