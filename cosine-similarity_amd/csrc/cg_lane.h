@@ -56,10 +56,12 @@ struct LaneCandidates {
 
     CS_HD void reset() { n = 0; }
 
-    // Insert (cv, cij); caller guarantees n < K.
-    CS_HD void insert(double cv, uint32_t cij) {
+    // Insert (cv, cij); caller guarantees n < K and n < bound <= K (bound as
+    // in greedy()).
+    CS_HD void insert(double cv, uint32_t cij, int bound = K) {
 #pragma unroll
         for (int s = 0; s < K; ++s) {
+            if (s >= bound) break;
             const bool take = (s >= n) || key_greater(cv, cij, v[s], ij[s]);
             const double tv = take ? v[s] : cv;
             const uint32_t tij = take ? ij[s] : cij;
@@ -71,13 +73,16 @@ struct LaneCandidates {
         ++n;
     }
 
-    // Greedy unique assignment over the sorted candidates.
-    CS_HD void greedy(double &score, int &matches) const {
+    // Greedy unique assignment over the sorted candidates.  `bound` >= n may
+    // be smaller than K; on the device it is the same for the whole wave, so
+    // the unrolled steps past it are skipped by a uniform branch.
+    CS_HD void greedy(double &score, int &matches, int bound = K) const {
         double s = 0.0;
         int m = 0;
         bool acc[K];
 #pragma unroll
         for (int a = 0; a < K; ++a) {
+            if (a >= bound) break;
             bool ok = a < n;
             const uint32_t ia = ij[a] >> 16, ja = ij[a] & 0xffffu;
 #pragma unroll

@@ -34,10 +34,10 @@ constexpr int kWave = 64;
 constexpr int kLightWaves = 8;
 constexpr int kLightThreads = kWave * kLightWaves;
 constexpr int kLightRefs = 64;        // references one light workgroup walks
-constexpr int kTileLdsCap = 5888;     // sorted tile peaks staged in LDS (larger: read from HBM)
+constexpr int kTileLdsCap = 6144;     // sorted tile peaks staged in LDS (larger: read from HBM)
 constexpr int kDefaultK = 8;
-constexpr int kDefaultRound = 512;
-constexpr int kHeavyGrid = 2048;
+constexpr int kDefaultRound = 256;
+constexpr int kHeavyGrid = 4096;
 constexpr int64_t kHeavyQueueCap = int64_t(1) << 20;
 constexpr int kMaxPeaks = 65535;
 
@@ -105,11 +105,50 @@ __global__ void k_tile_meta(const float *mz, const int64_t *off, int64_t nq, int
 
 // Merge the tile's sorted per-query lists into one list sorted by
 // (value, lane, peak): each element's rank is its own index plus, for every
-// other list, the count of elements ordered before it (binary search).
+// other list, the count of elements ordered before it (binary search).  The
+// tile's values are staged in LDS when they fit.
+constexpr int kSortStage = 8192;
+
+template <class X>
+__device__ void tile_rank_scatter(X xv, const int64_t *loff, int nl, int np, int64_t pb,
+                                  float *smz, uint32_t *swho) {
+    const int64_t g0 = loff[0];
+    for (int e = threadIdx.x; e < np; e += blockDim.x) {
+        const int64_t g = g0 + e;
+        int ql = 0;
+        while (ql + 1 < nl && loff[ql + 1] <= g) ++ql;
+        const int j = (int)(g - loff[ql]);
+        const float x = xv(e);
+        int64_t rank = j;
+        for (int l = 0; l < nl; ++l) {
+            if (l == ql) continue;
+            int a = (int)(loff[l] - g0), b = (int)(loff[l + 1] - g0);
+            const int base = a;
+            if (l < ql) {  // count y <= x
+                while (a < b) {
+                    const int m = (a + b) >> 1;
+                    if (xv(m) <= x) a = m + 1;
+                    else b = m;
+                }
+            } else {       // count y < x
+                while (a < b) {
+                    const int m = (a + b) >> 1;
+                    if (xv(m) < x) a = m + 1;
+                    else b = m;
+                }
+            }
+            rank += a - base;
+        }
+        smz[pb + rank] = x;
+        swho[pb + rank] = tile_who(ql, j);
+    }
+}
+
 __global__ void __launch_bounds__(256)
 k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, float shift32,
             const int32_t *tile_np, const int64_t *tile_pbase, float *smz, uint32_t *swho) {
     __shared__ int64_t loff[kTileQueries + 1];
+    __shared__ float xs[kSortStage];
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t q0 = t * kTileQueries;
         const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
@@ -117,35 +156,14 @@ k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, flo
         __syncthreads();
         const int np = tile_np[t];
         const int64_t pb = tile_pbase[t];
-        const int64_t g0 = loff[0];
-        for (int e = threadIdx.x; e < np; e += blockDim.x) {
-            const int64_t g = g0 + e;
-            int ql = 0;
-            while (ql + 1 < nl && loff[ql + 1] <= g) ++ql;
-            const int j = (int)(g - loff[ql]);
-            const float x = mz[g] + shift32;
-            int64_t rank = j;
-            for (int l = 0; l < nl; ++l) {
-                if (l == ql) continue;
-                int64_t a = loff[l], b = loff[l + 1];
-                const int64_t base = a;
-                if (l < ql) {  // count y <= x
-                    while (a < b) {
-                        const int64_t m = (a + b) >> 1;
-                        if (mz[m] + shift32 <= x) a = m + 1;
-                        else b = m;
-                    }
-                } else {       // count y < x
-                    while (a < b) {
-                        const int64_t m = (a + b) >> 1;
-                        if (mz[m] + shift32 < x) a = m + 1;
-                        else b = m;
-                    }
-                }
-                rank += a - base;
-            }
-            smz[pb + rank] = x;
-            swho[pb + rank] = tile_who(ql, j);
+        const float *src = mz + loff[0];
+        if (np <= kSortStage) {
+            for (int e = threadIdx.x; e < np; e += blockDim.x) xs[e] = src[e] + shift32;
+            __syncthreads();
+            tile_rank_scatter([&](int e) { return xs[e]; }, loff, nl, np, pb, smz, swho);
+        } else {
+            tile_rank_scatter([&](int e) { return src[e] + shift32; }, loff, nl, np, pb, smz,
+                              swho);
         }
         if (threadIdx.x == 0) {
             smz[pb + np] = __builtin_huge_valf();
@@ -199,17 +217,49 @@ struct LightArgs {
     unsigned long long *counters;
 };
 
+// Orders LDS accesses of the lanes of one wave.  LDS instructions of a wave
+// execute in issue order, so this only has to stop the compiler from moving
+// accesses across it (no s_waitcnt on outstanding global stores).
 __device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
 }
 
-// Bytes of dynamic LDS of k_light<K> for a tile capacity `cap`.
-constexpr size_t light_lds_bytes(int K, int cap) {
-    return (size_t)(cap + 1) * 8                                  // smz + swho
-           + ((size_t)(kTileBuckets + 1) * 2 + 15) / 16 * 16      // dir (u16)
-           + (size_t)kTileQueries * (8 + 8)                       // qoff, qnorm
-           + (size_t)kLightWaves * kTileQueries * (4 + 12 * K);   // cnt + slots
+// Per-wave candidate pool of k_light: every exact candidate of the current
+// reference against the tile, linked per query (LIFO; order is irrelevant,
+// the settle step sorts).
+constexpr int kPool = 256;
+constexpr int kMidCap = 64;   // candidates a wave settles for one query in registers
+constexpr size_t kWaveLds = (size_t)kPool * (8 + 4 + 4 + 1) + kTileQueries * 8 + 16 + kWave * 4;
+
+// Bytes of dynamic LDS of k_light for a tile capacity `cap`.
+constexpr size_t light_lds_bytes(int cap) {
+    return (size_t)kTileQueries * (8 + 8)                         // qoff, qnorm
+           + (size_t)kLightWaves * ((kWaveLds + 15) / 16 * 16)     // pools
+           + (size_t)(cap + 1) * 8                                 // smz + swho
+           + (size_t)(kTileBuckets + 1) * 2;                       // dir (u16)
+}
+
+// Descending bitonic sort of one key per lane across the wave (64 lanes).
+__device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const double ov = __shfl_xor(v, j);
+            const uint32_t oij = __shfl_xor(ij, j);
+            const bool lower = (lane & j) == 0;
+            const bool desc = (lane & k) == 0;
+            // the lower index of a descending block keeps the greater key
+            const bool keep_greater = lower == desc;
+            const bool other_greater = key_greater(ov, oij, v, ij);
+            const bool mine_greater = key_greater(v, ij, ov, oij);
+            const bool take = keep_greater ? other_greater : mine_greater;
+            v = take ? ov : v;
+            ij = take ? oij : ij;
+        }
+    }
 }
 
 template <int K>
@@ -217,17 +267,25 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     extern __shared__ __align__(16) unsigned char light_lds[];
     __shared__ int next_ref;
     const int cap = a.lds_cap;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     // LDS layout (see light_lds_bytes)
     int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
     double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
-    double *sv_all = qnorm_l + kTileQueries;
-    uint32_t *sij_all = reinterpret_cast<uint32_t *>(sv_all + kLightWaves * kTileQueries * K);
-    int *cnt_all = reinterpret_cast<int *>(sij_all + kLightWaves * kTileQueries * K);
-    float *smz_l = reinterpret_cast<float *>(cnt_all + kLightWaves * kTileQueries);
+    unsigned char *wbase = reinterpret_cast<unsigned char *>(qnorm_l + kTileQueries) +
+                           wave * ((kWaveLds + 15) / 16 * 16);
+    double *pv = reinterpret_cast<double *>(wbase);
+    uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
+    int *pnext = reinterpret_cast<int *>(pij + kPool);
+    int *cnt = pnext + kPool;
+    int *head = cnt + kTileQueries;
+    int *pool_n = head + kTileQueries;
+    int *midx = pool_n + 4;
+    unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
+    float *smz_l = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(qnorm_l + kTileQueries) +
+                                             kLightWaves * ((kWaveLds + 15) / 16 * 16));
     uint32_t *swho_l = reinterpret_cast<uint32_t *>(smz_l + cap + 1);
     uint16_t *dir_l = reinterpret_cast<uint16_t *>(swho_l + cap + 1);
 
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t rb = blockIdx.x, t = blockIdx.y;
     const int np = a.tile_np[t];
     const int64_t pb = a.tile_pbase[t];
@@ -250,10 +308,11 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         qnorm_l[tid] = a.Q.norm[q];
     }
     if (tid == 0) next_ref = 0;
-    int *cnt = cnt_all + wave * kTileQueries;
-    double *sv = sv_all + wave * kTileQueries * K;
-    uint32_t *sij = sij_all + wave * kTileQueries * K;
-    if (lane < kTileQueries) cnt[lane] = 0;
+    if (lane < kTileQueries) {
+        cnt[lane] = 0;
+        head[lane] = -1;
+    }
+    if (lane == 0) *pool_n = 0;
     __syncthreads();
 
     const int64_t r0 = rb * kLightRefs;
@@ -267,6 +326,8 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         if (r >= r1) break;
         const int64_t roff = R.off[r];
         const int nr = (int)(R.off[r + 1] - roff);
+
+        // 1. probe: lanes over reference peaks, exact candidates into the pool
         for (int ib = 0; ib < nr; ib += kWave) {
             const int i = ib + lane;
             if (i < nr) {
@@ -277,14 +338,14 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                     const int64_t qp = qoff_l[ql] + j;
                     const double mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qp);
                     const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
-                    if (exact_window(mr, mq, a.tol, a.shift)) {
-                        const double v = pair_value(R, roff + i, Q, qp);
-                        const int pos = atomicAdd(&cnt[ql], 1);
-                        if (pos < K) {
-                            sv[ql * K + pos] = v;
-                            sij[ql * K + pos] = pack_ij(i, j);
-                        }
-                    }
+                    if (!exact_window(mr, mq, a.tol, a.shift)) return;
+                    if (atomicAdd(&cnt[ql], 1) >= kMidCap) return;  // heavy anyway
+                    const int pos = atomicAdd(pool_n, 1);
+                    if (pos >= kPool) return;
+                    pv[pos] = pair_value(R, roff + i, Q, qp);
+                    pij[pos] = pack_ij(i, j);
+                    pq[pos] = (unsigned char)ql;
+                    pnext[pos] = atomicExch(&head[ql], pos);
                 };
                 if (in_lds) {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return smz_l[p]; },
@@ -298,41 +359,114 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
             }
         }
         wave_sync();
-        if (lane < kTileQueries) {
-            const int n = cnt[lane];
-            cnt[lane] = 0;
-            const int64_t q = q0 + lane;
-            if (q < a.nQ) {
-                const int64_t o = r * a.ld + q;
-                if (n > K) {
-                    const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
-                    if ((int64_t)slot < a.heavy_cap)
-                        a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
-                    a.score[o] = 0.0f;
-                    a.matches[o] = -1;
-                    a.overflow[o] = 0;
-                } else {
-                    LaneCandidates<K> c;
-                    c.reset();
-#pragma unroll
-                    for (int s = 0; s < K; ++s)
-                        if (s < n) c.insert(sv[lane * K + s], sij[lane * K + s]);
-                    double sc = 0.0;
-                    int m = 0;
-                    c.greedy(sc, m);
-                    a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[lane])) : 0.0f;
-                    a.matches[o] = m;
-                    a.overflow[o] = (uint8_t)(n >= a.match_limit);
-                }
+
+        // 2. settle each query of the tile
+        const int used = *pool_n;
+        const bool over = used > kPool;
+        const int n = lane < kTileQueries ? cnt[lane] : 0;
+        const int64_t q = q0 + lane;
+        const bool valid = lane < kTileQueries && q < a.nQ;
+        const bool complete = !(over && n > 0);  // every candidate of this query is pooled
+        // 2a. up to K candidates: walk the query's list into registers
+        const bool small = valid && complete && n <= K;
+        int bound = small ? n : 0;  // wave-wide max of the small counts
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const int o2 = __shfl_xor(bound, off);
+            bound = o2 > bound ? o2 : bound;
+        }
+        LaneCandidates<K> c;
+        c.reset();
+        int p = small ? head[lane] : -1;
+        while (__ballot(p >= 0)) {
+            if (p >= 0) {
+                c.insert(pv[p], pij[p], bound);
+                p = pnext[p];
             }
         }
+        if (small) {
+            const int64_t o = r * a.ld + q;
+            double sc = 0.0;
+            int m = 0;
+            c.greedy(sc, m, bound);
+            a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[lane])) : 0.0f;
+            a.matches[o] = m;
+            a.overflow[o] = (uint8_t)(n >= a.match_limit);
+        }
+        // 2b. K < n <= 64: the whole wave settles the query from registers
+        unsigned long long mid = __ballot(valid && complete && n > K && n <= kMidCap);
+        while (mid) {
+            const int ql = __builtin_ctzll(mid);
+            mid &= mid - 1ull;
+            int have = 0;
+            const int lim = used < kPool ? used : kPool;
+            for (int base = 0; base < lim; base += kWave) {  // compact the query's entries
+                const int e = base + lane;
+                const bool mine = e < lim && pq[e] == ql;
+                const unsigned long long mk = __ballot(mine);
+                // at most kMidCap entries belong to ql, so every slot is < 64
+                if (mine) midx[have + __popcll(mk & ((1ull << lane) - 1ull))] = e;
+                have += __popcll(mk);
+            }
+            wave_sync();
+            double v = -__builtin_huge_val();
+            uint32_t ij = 0u;
+            if (lane < have) {
+                const int e = midx[lane];
+                v = pv[e];
+                ij = pij[e];
+            }
+            wave_sync();
+            wave_sort_desc(v, ij);
+            const int vi = (int)(ij >> 16), vj = (int)(ij & 0xffffu);
+            bool alive = lane < have;
+            double sc = 0.0;
+            int m = 0;
+            unsigned long long mk = __ballot(alive);
+            while (mk) {
+                const int f = __builtin_ctzll(mk);
+                const int fi = __shfl(vi, f), fj = __shfl(vj, f);
+                sc += __shfl(v, f);
+                ++m;
+                alive = alive && lane != f && vi != fi && vj != fj;
+                mk = __ballot(alive);
+            }
+            if (lane == 0) {
+                const int64_t o = r * a.ld + q0 + ql;
+                const int nn = cnt[ql];
+                a.score[o] = (float)(sc / (R.norm[r] * qnorm_l[ql]));
+                a.matches[o] = m;
+                a.overflow[o] = (uint8_t)(nn >= a.match_limit);
+            }
+        }
+        // 2c. the rest go to the heavy kernel
+        if (valid && (n > kMidCap || (!complete))) {
+            const int64_t o = r * a.ld + q;
+            const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
+            if ((int64_t)slot < a.heavy_cap)
+                a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
+            a.score[o] = 0.0f;
+            a.matches[o] = -1;
+            a.overflow[o] = 0;
+        }
+        wave_sync();
+        if (lane < kTileQueries) {
+            cnt[lane] = 0;
+            head[lane] = -1;
+        }
+        if (lane == 0) *pool_n = 0;
         wave_sync();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Heavy kernel: one wave per pair with more than K candidates
+// Heavy kernel: one workgroup per pair with more than 64 candidates
 // ---------------------------------------------------------------------------
+
+constexpr int kHeavyThreads = 256;
+constexpr int kHeavyBuf = 4096;     // candidates one pair settles in a single pass
+constexpr int kHeavyStage = 1024;   // peaks per spectrum staged in LDS
+constexpr int kHeavyBlocks = 1024;
+constexpr int kGiantGrid = 512;
 
 struct HeavyArgs {
     SpecView R, Q;
@@ -340,16 +474,19 @@ struct HeavyArgs {
     float tolc, shift32;
     double tol, shift;
     int64_t match_limit;
-    int round_cap;     // T
-    int buf_cap;       // next_pow2(2T + 64) entries
-    int r_words, q_words;
+    int round_cap;         // giant path: candidates settled per round (T)
+    int buf_cap;           // giant path: next_pow2(2T + 64) entries
+    int r_words, q_words;  // bitset words for the largest spectra
+    int stage;             // both spectra fit kHeavyStage: staged in LDS
+    int exact_from_stage;  // staged query m/z are the exact values (no shift, no float64)
     float *score;
     int32_t *matches;
     uint8_t *overflow;
     int64_t ld;
     const unsigned long long *heavy_queue;
     int64_t heavy_cap;
-    unsigned long long *counters;
+    unsigned long long *counters;  // [0] heavy count [1] heavy next [2] giant count [3] giant next
+    unsigned long long *giant_queue;
 };
 
 __device__ __forceinline__ bool bit_set(const uint32_t *b, int x) {
@@ -357,19 +494,19 @@ __device__ __forceinline__ bool bit_set(const uint32_t *b, int x) {
 }
 
 // Pad [n, next_pow2(n)) with -inf keys and sort the first next_pow2(n)
-// entries descending by key_greater.  One wave, LDS.
+// entries descending by key_greater, with every thread of the block.
 __device__ void sort_desc(double *bv, uint32_t *bij, int n) {
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
     int n2 = 1;
     while (n2 < n) n2 <<= 1;
-    for (int e = n + lane; e < n2; e += kWave) {
+    for (int e = n + tid; e < n2; e += nt) {
         bv[e] = -__builtin_huge_val();
         bij[e] = 0u;
     }
     __syncthreads();
     for (int size = 2; size <= n2; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = lane; t < (n2 >> 1); t += kWave) {
+            for (int t = tid; t < (n2 >> 1); t += nt) {
                 const int lo = 2 * t - (t & (stride - 1));
                 const int hi = lo + stride;
                 const bool desc = (lo & size) == 0;
@@ -388,12 +525,13 @@ __device__ void sort_desc(double *bv, uint32_t *bij, int n) {
     }
 }
 
-// Greedy over sorted entries [0, n): chunks of 64 in order; inside a chunk the
-// first live candidate is accepted and every later one sharing its i or j is
-// dropped, repeatedly, which is exactly the sequential greedy.
+// Greedy over sorted entries [0, n), run by one wave: chunks of 64 in order;
+// inside a chunk the first live candidate is accepted and every later one
+// sharing its i or j is dropped, repeatedly, which is exactly the sequential
+// greedy.  The accepted sum runs in acceptance order.
 __device__ void greedy_sorted(const double *bv, const uint32_t *bij, int n, uint32_t *used_r,
                               uint32_t *used_q, double &s, int &m) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
     for (int base = 0; base < n; base += kWave) {
         const int idx = base + lane;
         const bool valid = idx < n;
@@ -415,23 +553,159 @@ __device__ void greedy_sorted(const double *bv, const uint32_t *bij, int n, uint
             alive = alive && lane != f && i != fi && j != fj;
             mask = __ballot(alive);
         }
-        __syncthreads();
+        wave_sync();
     }
 }
 
-__device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, uint32_t *used_r,
+__device__ void write_error(const HeavyArgs &a, int64_t o) {
+    a.score[o] = __builtin_nanf("");
+    a.matches[o] = -2;
+    a.overflow[o] = 1;
+}
+
+// One pair, whole block: stage both spectra, enumerate every candidate once
+// into LDS, sort, greedy.  More than kHeavyBuf candidates: queued as giant.
+__device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, unsigned char *lds,
+                           int *bn_s) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const SpecView &R = a.R, &Q = a.Q;
+    const int64_t roff = R.off[r], qoff = Q.off[q];
+    const int nr = (int)(R.off[r + 1] - roff), nq = (int)(Q.off[q + 1] - qoff);
+    const int64_t o = r * a.ld + q;
+    const int S = a.stage ? kHeavyStage : 0;
+    double *bv = reinterpret_cast<double *>(lds);
+    double *r_w = bv + kHeavyBuf;
+    double *q_d = r_w + S;
+    uint32_t *bij = reinterpret_cast<uint32_t *>(q_d + S);
+    float *r_mz = reinterpret_cast<float *>(bij + kHeavyBuf);
+    float *q_ms = r_mz + S;
+    uint32_t *used_r = reinterpret_cast<uint32_t *>(q_ms + S);
+    uint32_t *used_q = used_r + a.r_words;
+    const int nrw = (nr + 31) >> 5, nqw = (nq + 31) >> 5;
+    if (nrw > a.r_words || nqw > a.q_words || (a.stage && (nr > S || nq > S))) {
+        if (tid == 0) write_error(a, o);  // caller's max_peaks was wrong
+        return;
+    }
+    if (a.stage) {
+        for (int k = tid; k < nr; k += nt) {
+            r_mz[k] = R.mz[roff + k];
+            r_w[k] = (R.mzp ? R.mzp[roff + k] : 1.0) * R.inp[roff + k];
+        }
+        for (int k = tid; k < nq; k += nt) {
+            q_ms[k] = Q.mz[qoff + k] + a.shift32;
+            q_d[k] = Q.inp[qoff + k];
+        }
+    }
+    for (int w = tid; w < nrw; w += nt) used_r[w] = 0u;
+    for (int w = tid; w < nqw; w += nt) used_q[w] = 0u;
+    if (tid == 0) *bn_s = 0;
+    __syncthreads();
+
+    auto rmz = [&](int i) { return a.stage ? r_mz[i] : R.mz[roff + i]; };
+    auto qms = [&](int j) { return a.stage ? q_ms[j] : Q.mz[qoff + j] + a.shift32; };
+    for (int i = tid; i < nr; i += nt) {
+        const float ri = rmz(i);
+        int lo = 0, hi = nq;  // first query peak not above the window
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ri - qms(mid) > a.tolc) lo = mid + 1;
+            else hi = mid;
+        }
+        const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+        const double wr = a.stage ? r_w[i] : (R.mzp ? R.mzp[roff + i] : 1.0) * R.inp[roff + i];
+        for (int j = lo; j < nq; ++j) {
+            const float qj = qms(j);
+            if (ri - qj < -a.tolc) break;
+            const double mq = a.exact_from_stage ? (double)qj : exact_mz(Q, qoff + j);
+            if (!exact_window(mr, mq, a.tol, a.shift)) continue;
+            double v = wr * (Q.mzp ? Q.mzp[qoff + j] : 1.0);
+            v = v * (a.stage ? q_d[j] : Q.inp[qoff + j]);
+            const int pos = atomicAdd(bn_s, 1);
+            if (pos < kHeavyBuf) {
+                bv[pos] = v;
+                bij[pos] = pack_ij(i, j);
+            }
+        }
+    }
+    __syncthreads();
+    const int C = *bn_s;
+    if (C > kHeavyBuf) {
+        if (tid == 0) {
+            const unsigned long long slot = atomicAdd(&a.counters[2], 1ull);
+            if ((int64_t)slot < a.heavy_cap)
+                a.giant_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
+        }
+        __syncthreads();
+        return;
+    }
+    sort_desc(bv, bij, C);
+    if (tid < kWave) {
+        double sc = 0.0;
+        int m = 0;
+        greedy_sorted(bv, bij, C, used_r, used_q, sc, m);
+        if (tid == 0) {
+            a.score[o] = (float)(sc / (R.norm[r] * Q.norm[q]));
+            a.matches[o] = m;
+            a.overflow[o] = (uint8_t)(C >= a.match_limit);
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kHeavyThreads) k_heavy(HeavyArgs a) {
+    extern __shared__ __align__(16) unsigned char heavy_lds[];
+    __shared__ int bn_s;
+    __shared__ unsigned long long item_s;
+    __shared__ int nlist;
+    __shared__ long long list[kHeavyThreads];
+    const unsigned long long count = a.counters[0];
+    if (count == 0) return;
+    const int tid = threadIdx.x;
+    if ((int64_t)count <= a.heavy_cap) {
+        for (;;) {
+            if (tid == 0) item_s = atomicAdd(&a.counters[1], 1ull);
+            __syncthreads();
+            const unsigned long long item = item_s;
+            __syncthreads();
+            if (item >= count) break;
+            const unsigned long long rq = a.heavy_queue[item];
+            heavy_pair(a, (int64_t)(rq >> 32), (int64_t)(rq & 0xffffffffull), heavy_lds, &bn_s);
+        }
+    } else {  // queue overflowed: find the -1 sentinels
+        const int64_t total = a.nR * a.nQ;
+        for (int64_t c = blockIdx.x; c * kHeavyThreads < total; c += gridDim.x) {
+            if (tid == 0) nlist = 0;
+            __syncthreads();
+            const int64_t p = c * kHeavyThreads + tid;
+            if (p < total) {
+                const int64_t r = p / a.nQ, q = p - r * a.nQ;
+                if (a.matches[r * a.ld + q] == -1) list[atomicAdd(&nlist, 1)] = p;
+            }
+            __syncthreads();
+            const int n = nlist;
+            for (int k = 0; k < n; ++k) {
+                const int64_t pp = list[k];
+                heavy_pair(a, pp / a.nQ, pp % a.nQ, heavy_lds, &bn_s);
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Giant pairs (more than kHeavyBuf candidates): one wave per pair, in rounds.
+// A round collects the best T alive candidates below the previous round's last
+// key (a streaming top-T with a rising cut), settles them in order, and
+// continues, so LDS stays bounded for any candidate count and the result is
+// exactly the sequential greedy.
+__device__ void giant_pair(const HeavyArgs &a, int64_t r, int64_t q, uint32_t *used_r,
                            uint32_t *used_q, double *bv, uint32_t *bij) {
     const int lane = threadIdx.x;
     const int64_t roff = a.R.off[r], qoff = a.Q.off[q];
     const int nr = (int)(a.R.off[r + 1] - roff), nq = (int)(a.Q.off[q + 1] - qoff);
     const int nrw = (nr + 31) >> 5, nqw = (nq + 31) >> 5;
     const int64_t o = r * a.ld + q;
-    if (nrw > a.r_words || nqw > a.q_words) {  // caller's max_peaks was wrong
-        if (lane == 0) {
-            a.score[o] = __builtin_nanf("");
-            a.matches[o] = -2;
-            a.overflow[o] = 1;
-        }
+    if (nrw > a.r_words || nqw > a.q_words) {
+        if (lane == 0) write_error(a, o);
         return;
     }
     for (int w = lane; w < nrw; w += kWave) used_r[w] = 0u;
@@ -458,12 +732,11 @@ __device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, uint32_t *u
             bool act = i < nr;
             const float ri = act ? rmz[i] : 0.0f;
             int j = nq;
-            if (act) {  // first query peak not above the window of reference peak i
+            if (act) {
                 int lo = 0, hi = nq;
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
-                    const float d = ri - (qmz[mid] + a.shift32);
-                    if (d > a.tolc) lo = mid + 1;
+                    if (ri - (qmz[mid] + a.shift32) > a.tolc) lo = mid + 1;
                     else hi = mid;
                 }
                 j = lo;
@@ -517,6 +790,7 @@ __device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, uint32_t *u
         const double last_v = take > 0 ? bv[take - 1] : 0.0;
         const uint32_t last_ij = take > 0 ? bij[take - 1] : 0u;
         greedy_sorted(bv, bij, take, used_r, used_q, s, m);
+        __syncthreads();
         if (!compacted) break;
         have_up = true;
         up_v = last_v;
@@ -532,27 +806,27 @@ __device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, uint32_t *u
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kWave) k_heavy(HeavyArgs a) {
-    extern __shared__ __align__(16) unsigned char heavy_lds[];
-    double *bv = reinterpret_cast<double *>(heavy_lds);
+__global__ void __launch_bounds__(kWave) k_giant(HeavyArgs a) {
+    extern __shared__ __align__(16) unsigned char giant_lds[];
+    double *bv = reinterpret_cast<double *>(giant_lds);
     uint32_t *bij = reinterpret_cast<uint32_t *>(bv + a.buf_cap);
     uint32_t *used_r = bij + a.buf_cap;
     uint32_t *used_q = used_r + a.r_words;
 
-    const unsigned long long count = a.counters[0];
+    const unsigned long long count = a.counters[2];
     if (count == 0) return;
     const int lane = threadIdx.x;
     if ((int64_t)count <= a.heavy_cap) {
         for (;;) {
             unsigned long long item = 0;
-            if (lane == 0) item = atomicAdd(&a.counters[1], 1ull);
+            if (lane == 0) item = atomicAdd(&a.counters[3], 1ull);
             item = __shfl(item, 0);
             if (item >= count) break;
-            const unsigned long long rq = a.heavy_queue[item];
-            heavy_pair(a, (int64_t)(rq >> 32), (int64_t)(rq & 0xffffffffull), used_r, used_q, bv,
+            const unsigned long long rq = a.giant_queue[item];
+            giant_pair(a, (int64_t)(rq >> 32), (int64_t)(rq & 0xffffffffull), used_r, used_q, bv,
                        bij);
         }
-    } else {  // queue overflowed: find the -1 sentinels
+    } else {  // after k_heavy, the pairs still at -1 are exactly the giant ones
         const int64_t total = a.nR * a.nQ;
         for (int64_t c = blockIdx.x; c * kWave < total; c += gridDim.x) {
             const int64_t p = c * kWave + lane;
@@ -567,7 +841,7 @@ __global__ void __launch_bounds__(kWave) k_heavy(HeavyArgs a) {
             while (mask) {
                 const int f = __builtin_ctzll(mask);
                 const int64_t rf = __shfl(r, f), qf = __shfl(q, f);
-                heavy_pair(a, rf, qf, used_r, used_q, bv, bij);
+                giant_pair(a, rf, qf, used_r, used_q, bv, bij);
                 mask &= mask - 1ull;
             }
         }
@@ -584,7 +858,7 @@ struct Layout {
     float *tile_lo, *tile_iw, *smz;
     uint32_t *swho;
     int32_t *dir;
-    unsigned long long *counters, *heavy_queue;
+    unsigned long long *counters, *heavy_queue, *giant_queue;
     int64_t ntiles, heavy_cap;
     size_t bytes;
 };
@@ -610,6 +884,7 @@ Layout layout(void *base, const cs_spectra &R, const cs_spectra &Q) {
     L.dir = c.take<int32_t>(T * (kTileBuckets + 1));
     L.counters = c.take<unsigned long long>(4);
     L.heavy_queue = c.take<unsigned long long>(L.heavy_cap);
+    L.giant_queue = c.take<unsigned long long>(L.heavy_cap);
     L.bytes = align_up(c.used, 256);
     return L;
 }
@@ -642,7 +917,7 @@ int light_capacity(int K) {
             v = 65536;
         max_lds = v;
     }
-    const size_t fixed = light_lds_bytes(K, 0) + 64;  // + static next_ref and slack
+    const size_t fixed = light_lds_bytes(0) + 64;  // + static next_ref and slack
     const size_t room = (size_t)max_lds > fixed ? (size_t)max_lds - fixed : 0;
     int cap = (int)(room / 8) - 1;
     // two workgroups per CU when the CU has 160 KiB
@@ -655,7 +930,7 @@ int light_capacity(int K) {
 template <int K>
 cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
     a.lds_cap = light_capacity(K);
-    const size_t lds = light_lds_bytes(K, a.lds_cap);
+    const size_t lds = light_lds_bytes(a.lds_cap);
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_light<K>),
@@ -800,6 +1075,8 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     a.buf_cap = cap;
     a.r_words = (Rs.max_peaks + 31) / 32 + 1;
     a.q_words = (Qs.max_peaks + 31) / 32 + 1;
+    a.stage = (Rs.max_peaks <= kHeavyStage && Qs.max_peaks <= kHeavyStage) ? 1 : 0;
+    a.exact_from_stage = (Qs.mz_exact == nullptr && shift == 0.0) ? 1 : 0;
     a.score = score;
     a.matches = matches;
     a.overflow = overflow;
@@ -807,13 +1084,29 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     a.heavy_queue = L.heavy_queue;
     a.heavy_cap = L.heavy_cap;
     a.counters = L.counters;
-    const size_t lds = (size_t)cap * (sizeof(double) + sizeof(uint32_t)) +
-                       sizeof(uint32_t) * (size_t)(a.r_words + a.q_words);
-    if (lds > 65536)
+    a.giant_queue = L.giant_queue;
+    const int S = a.stage ? kHeavyStage : 0;
+    const size_t hlds = (size_t)(kHeavyBuf + 2 * S) * 12 +
+                        sizeof(uint32_t) * (size_t)(a.r_words + a.q_words);
+    if (hlds > 160 * 1024)
+        return fail(CS_ERR_UNSUPPORTED, "max_peaks %d/%d needs %zu bytes of LDS", Rs.max_peaks,
+                    Qs.max_peaks, hlds);
+    static size_t heavy_attr = 0;
+    if (hlds > heavy_attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_heavy),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)hlds);
+        if (e != hipSuccess) return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+        heavy_attr = hlds;
+    }
+    hipLaunchKernelGGL(k_heavy, dim3(kHeavyBlocks), dim3(kHeavyThreads), hlds, hs, a);
+    CS_LAUNCH_CHECK("k_heavy");
+    const size_t glds = (size_t)cap * (sizeof(double) + sizeof(uint32_t)) +
+                        sizeof(uint32_t) * (size_t)(a.r_words + a.q_words);
+    if (glds > 65536)
         return fail(CS_ERR_UNSUPPORTED,
                     "round_capacity %d with max_peaks %d/%d needs %zu bytes of LDS (> 64 KiB)", T,
-                    Rs.max_peaks, Qs.max_peaks, lds);
-    hipLaunchKernelGGL(k_heavy, dim3(kHeavyGrid), dim3(kWave), lds, hs, a);
-    CS_LAUNCH_CHECK("k_heavy");
+                    Rs.max_peaks, Qs.max_peaks, glds);
+    hipLaunchKernelGGL(k_giant, dim3(kGiantGrid), dim3(kWave), glds, hs, a);
+    CS_LAUNCH_CHECK("k_giant");
     return CS_OK;
 }

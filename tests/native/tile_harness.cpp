@@ -13,6 +13,8 @@
 
 namespace {
 
+constexpr int kMid = 64;  // candidates the device settles per query without the heavy kernel
+
 struct Entry {
     float x;
     int lane, j;
@@ -61,8 +63,8 @@ void run(const double *rmz, const double *rint, const int64_t *roff, int R, cons
         }
         for (int r = 0; r < R; ++r) {
             std::vector<int> cnt(nl, 0);
-            std::vector<double> sv(nl * K);
-            std::vector<uint32_t> sij(nl * K);
+            std::vector<double> sv(nl * kMid);
+            std::vector<uint32_t> sij(nl * kMid);
             for (int64_t g = roff[r]; g < roff[r + 1]; ++g) {
                 const int i = (int)(g - roff[r]);
                 const float ri = (float)rmz[g];
@@ -75,9 +77,9 @@ void run(const double *rmz, const double *rint, const int64_t *roff, int R, cons
                                    v = v * wmz(qmz[qp]);
                                    v = v * pow(qint[qp], inp);
                                    const int pos = cnt[ql]++;
-                                   if (pos < K) {
-                                       sv[ql * K + pos] = v;
-                                       sij[ql * K + pos] = cs::pack_ij(i, j);
+                                   if (pos < kMid) {
+                                       sv[ql * kMid + pos] = v;
+                                       sij[ql * kMid + pos] = cs::pack_ij(i, j);
                                    }
                                });
             }
@@ -85,18 +87,37 @@ void run(const double *rmz, const double *rint, const int64_t *roff, int R, cons
                 const int64_t o = (int64_t)r * Q + q0 + l;
                 const int n = cnt[l];
                 candidates[o] = n;
-                heavy[o] = n > K;
-                if (n > K) {
+                heavy[o] = n > kMid;
+                if (n > kMid) {
                     score[o] = 0.0;
                     matches[o] = -1;
                     continue;
                 }
-                cs::LaneCandidates<K> c;
-                c.reset();
-                for (int s = 0; s < n; ++s) c.insert(sv[l * K + s], sij[l * K + s]);
                 double sc = 0.0;
                 int m = 0;
-                c.greedy(sc, m);
+                if (n <= K) {  // the lane path: register insertion + greedy
+                    cs::LaneCandidates<K> c;
+                    c.reset();
+                    for (int s = 0; s < n; ++s) c.insert(sv[l * kMid + s], sij[l * kMid + s]);
+                    c.greedy(sc, m);
+                } else {       // the wave path: sort by key, then sequential greedy
+                    std::vector<std::pair<double, uint32_t>> e;
+                    for (int s = 0; s < n; ++s) e.push_back({sv[l * kMid + s], sij[l * kMid + s]});
+                    std::sort(e.begin(), e.end(), [](const auto &a, const auto &b) {
+                        return cs::key_greater(a.first, a.second, b.first, b.second);
+                    });
+                    std::vector<uint32_t> ui, uj;
+                    for (auto &x : e) {
+                        const uint32_t a = x.second >> 16, b = x.second & 0xffffu;
+                        if (std::find(ui.begin(), ui.end(), a) != ui.end() ||
+                            std::find(uj.begin(), uj.end(), b) != uj.end())
+                            continue;
+                        ui.push_back(a);
+                        uj.push_back(b);
+                        sc += x.first;
+                        ++m;
+                    }
+                }
                 score[o] = n ? sc / (norm(rmz, rint, roff[r], roff[r + 1]) *
                                      norm(qmz, qint, qoff[q0 + l], qoff[q0 + l + 1]))
                              : 0.0;

@@ -322,3 +322,25 @@ def test_stage_reuse_with_other_references(gpu, oracle):
         torch.cuda.synchronize()
         res = {"score": s.cpu().numpy(), "matches": m.cpu().numpy(), "overflow": o.cpu().numpy()}
         assert_parity(res, oracle_csr(oracle, refs, lib, tolerance=0.01))
+
+
+@pytest.mark.parametrize("T", [4, 64, 1024])
+def test_giant_pairs(gpu, oracle, T):
+    """Pairs with more candidates than the single-pass buffer (4096): dense
+    peak packs inside a wide tolerance give 5k-40k candidates per pair; the
+    round-based path settles them T at a time."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(77)
+    mzs, its = [], []
+    for x in range(9):
+        k = int(rng.integers(60, 200))
+        base = 300.0 + 40.0 * (x % 3)
+        mzs.append(np.sort(base + rng.uniform(0, 0.8, k).astype(np.float32)).astype(np.float64))
+        its.append(np.round(rng.random(k) * 8 + 1) / 8)  # ties
+    p = PackedSpectra.from_arrays(mzs, its)
+    say(f"giant pairs T={T}")
+    res = device_run(p, p, tol=1.0, T=T, match_limit=5000)
+    want = oracle_csr(oracle, p, p, tolerance=1.0, match_limit=5000)
+    assert want["n_candidates"].max() > 4096
+    assert_parity(res, want, 5000)
