@@ -33,13 +33,14 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kLightWaves = 8;
 constexpr int kLightThreads = kWave * kLightWaves;
-constexpr int kLightRefs = 64;        // references one light workgroup walks
+constexpr int kLightRefs = 128;       // references one light workgroup walks
 constexpr int kTileLdsCap = 6144;     // sorted tile peaks staged in LDS (larger: read from HBM)
 constexpr int kDefaultK = 8;
 constexpr int kDefaultRound = 256;
 constexpr int kHeavyGrid = 4096;
 constexpr int64_t kHeavyQueueCap = int64_t(1) << 20;
 constexpr int kMaxPeaks = 65535;
+constexpr int kIndexStage = 6144;     // tiles up to this many peaks are indexed in LDS
 
 struct SpecView {
     const int64_t *off;
@@ -155,6 +156,10 @@ k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, flo
         if ((int)threadIdx.x <= nl) loff[threadIdx.x] = off[q0 + threadIdx.x];
         __syncthreads();
         const int np = tile_np[t];
+        if (np <= kIndexStage) {  // uniform: k_tile_index handles it
+            __syncthreads();
+            continue;
+        }
         const int64_t pb = tile_pbase[t];
         const float *src = mz + loff[0];
         if (np <= kSortStage) {
@@ -173,12 +178,99 @@ k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, flo
     }
 }
 
+// Tile index in LDS for tiles of at most kIndexStage peaks: counting sort by
+// bucket (the exclusive prefix of the bucket histogram is the directory),
+// then each element's final place inside its bucket is its rank by
+// (value, lane, peak) among the bucket's elements.  Larger tiles are marked
+// in `big` and indexed by k_tile_sort + k_tile_dir.
+__global__ void __launch_bounds__(256)
+k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, float shift32,
+             const int32_t *tile_np, const int64_t *tile_pbase, const float *tile_lo,
+             const float *tile_iw, float *smz, uint32_t *swho, int32_t *dir) {
+    __shared__ int64_t loff[kTileQueries + 1];
+    __shared__ int start[kTileBuckets + 1];
+    __shared__ int cursor[kTileBuckets];
+    __shared__ float bx[kIndexStage];
+    __shared__ uint32_t bw[kIndexStage];
+    __shared__ int part[256];
+    const int tid = threadIdx.x;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int np = tile_np[t];
+        if (np > kIndexStage) continue;  // uniform: left to the rank-merge path
+        const int64_t q0 = t * kTileQueries;
+        const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
+        if (tid <= nl) loff[tid] = off[q0 + tid];
+        for (int b = tid; b <= kTileBuckets; b += 256) start[b] = 0;
+        __syncthreads();
+        const float lo = tile_lo[t], iw = tile_iw[t];
+        const int64_t g0 = loff[0];
+        for (int e = tid; e < np; e += 256)
+            atomicAdd(&start[tile_bucket(mz[g0 + e] + shift32, lo, iw)], 1);
+        __syncthreads();
+        // exclusive scan of start[0..kTileBuckets) (8 buckets per thread)
+        int loc[kTileBuckets / 256];
+        int sum = 0;
+#pragma unroll
+        for (int u = 0; u < kTileBuckets / 256; ++u) {
+            loc[u] = start[tid * (kTileBuckets / 256) + u];
+            sum += loc[u];
+        }
+        part[tid] = sum;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int v = tid >= o ? part[tid - o] : 0;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
+        }
+        int run = tid ? part[tid - 1] : 0;
+#pragma unroll
+        for (int u = 0; u < kTileBuckets / 256; ++u) {
+            const int b = tid * (kTileBuckets / 256) + u;
+            start[b] = run;
+            cursor[b] = run;
+            run += loc[u];
+        }
+        if (tid == 0) start[kTileBuckets] = np;
+        __syncthreads();
+        int32_t *d = dir + t * (kTileBuckets + 1);
+        for (int b = tid; b <= kTileBuckets; b += 256) d[b] = start[b];
+        for (int e = tid; e < np; e += 256) {  // scatter into buckets
+            const int64_t g = g0 + e;
+            int ql = 0;
+            while (ql + 1 < nl && loff[ql + 1] <= g) ++ql;
+            const float x = mz[g] + shift32;
+            const int pos = atomicAdd(&cursor[tile_bucket(x, lo, iw)], 1);
+            bx[pos] = x;
+            bw[pos] = tile_who(ql, (int)(g - loff[ql]));
+        }
+        __syncthreads();
+        const int64_t pb = tile_pbase[t];
+        for (int e = tid; e < np; e += 256) {  // rank inside the bucket
+            const float x = bx[e];
+            const uint32_t w = bw[e];
+            const int b = tile_bucket(x, lo, iw);
+            int rank = start[b];
+            for (int k = start[b]; k < start[b + 1]; ++k)
+                rank += (bx[k] < x || (bx[k] == x && bw[k] < w)) ? 1 : 0;
+            smz[pb + rank] = x;
+            swho[pb + rank] = w;
+        }
+        if (tid == 0) {
+            smz[pb + np] = __builtin_huge_valf();
+            swho[pb + np] = 0u;
+        }
+        __syncthreads();
+    }
+}
+
 // dir[b] = first list position whose bucket is >= b, for b in [0, kTileBuckets].
 __global__ void __launch_bounds__(256)
 k_tile_dir(int64_t ntiles, const int32_t *tile_np, const int64_t *tile_pbase,
            const float *tile_lo, const float *tile_iw, const float *smz, int32_t *dir) {
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int np = tile_np[t];
+        if (np <= kIndexStage) continue;
         const float *s = smz + tile_pbase[t];
         const float lo = tile_lo[t], iw = tile_iw[t];
         int32_t *d = dir + t * (kTileBuckets + 1);
@@ -230,7 +322,7 @@ __device__ __forceinline__ void wave_sync() {
 // the settle step sorts).
 constexpr int kPool = 256;
 constexpr int kMidCap = 64;   // candidates a wave settles for one query in registers
-constexpr size_t kWaveLds = (size_t)kPool * (8 + 4 + 4 + 1) + kTileQueries * 8 + 16 + kWave * 4;
+constexpr size_t kWaveLds = (size_t)kPool * (8 + 4 + 4 + 1) + 2 * kTileQueries * 8 + 16 + kWave * 4;
 
 // Bytes of dynamic LDS of k_light for a tile capacity `cap`.
 constexpr size_t light_lds_bytes(int cap) {
@@ -262,6 +354,12 @@ __device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
     }
 }
 
+// A wave takes two references at a time: their peaks form one probe stream
+// (fewer idle lanes in the last round), and in the settle step lanes 0-31
+// settle the first reference's pairs with the tile's 32 queries and lanes
+// 32-63 the second's.  Pool slots are keyed by s = half * 32 + query lane.
+constexpr int kSlots = 2 * kTileQueries;
+
 template <int K>
 __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     extern __shared__ __align__(16) unsigned char light_lds[];
@@ -277,8 +375,8 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
     int *pnext = reinterpret_cast<int *>(pij + kPool);
     int *cnt = pnext + kPool;
-    int *head = cnt + kTileQueries;
-    int *pool_n = head + kTileQueries;
+    int *head = cnt + kSlots;
+    int *pool_n = head + kSlots;
     int *midx = pool_n + 4;
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
     float *smz_l = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(qnorm_l + kTileQueries) +
@@ -295,12 +393,36 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     const int32_t *gdir = a.dir + t * (kTileBuckets + 1);
     const bool in_lds = np <= cap;
     const int64_t q0 = t * kTileQueries;
-    if (in_lds) {
-        for (int e = tid; e <= np; e += kLightThreads) {
-            smz_l[e] = gsmz[e];
-            swho_l[e] = gswho[e];
+    if (in_lds) {  // four independent loads in flight per thread and array
+        for (int e = tid; e <= np; e += 4 * kLightThreads) {
+            float x[4];
+            uint32_t w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int f = e + u * kLightThreads;
+                x[u] = f <= np ? gsmz[f] : 0.0f;
+                w[u] = f <= np ? gswho[f] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int f = e + u * kLightThreads;
+                if (f <= np) {
+                    smz_l[f] = x[u];
+                    swho_l[f] = w[u];
+                }
+            }
         }
-        for (int b = tid; b <= kTileBuckets; b += kLightThreads) dir_l[b] = (uint16_t)gdir[b];
+        int32_t d[5];
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            const int b = tid + u * kLightThreads;
+            d[u] = b <= kTileBuckets ? gdir[b] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            const int b = tid + u * kLightThreads;
+            if (b <= kTileBuckets) dir_l[b] = (uint16_t)d[u];
+        }
     }
     if (tid < kTileQueries) {
         const int64_t q = q0 + tid < a.nQ ? q0 + tid : q0;
@@ -308,29 +430,37 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         qnorm_l[tid] = a.Q.norm[q];
     }
     if (tid == 0) next_ref = 0;
-    if (lane < kTileQueries) {
-        cnt[lane] = 0;
-        head[lane] = -1;
-    }
+    cnt[lane] = 0;
+    head[lane] = -1;
     if (lane == 0) *pool_n = 0;
     __syncthreads();
 
     const int64_t r0 = rb * kLightRefs;
     const int64_t r1 = r0 + kLightRefs < a.nR ? r0 + kLightRefs : a.nR;
     const SpecView &R = a.R, &Q = a.Q;
+    const int half = lane >> 5, ql_mine = lane & 31;
+    const int64_t q_mine = q0 + ql_mine;
     for (;;) {
         int rr = 0;
-        if (lane == 0) rr = atomicAdd(&next_ref, 1);
+        if (lane == 0) rr = atomicAdd(&next_ref, 2);
         rr = __shfl(rr, 0);
-        const int64_t r = r0 + rr;
-        if (r >= r1) break;
-        const int64_t roff = R.off[r];
-        const int nr = (int)(R.off[r + 1] - roff);
+        const int64_t ra = r0 + rr;
+        if (ra >= r1) break;
+        const bool has_b = ra + 1 < r1;
+        const int64_t rb2 = has_b ? ra + 1 : ra;
+        const int64_t roff_a = R.off[ra], roff_b = R.off[rb2];
+        const int nr_a = (int)(R.off[ra + 1] - roff_a);
+        const int nr_b = has_b ? (int)(R.off[rb2 + 1] - roff_b) : 0;
+        const int total = nr_a + nr_b;
 
-        // 1. probe: lanes over reference peaks, exact candidates into the pool
-        for (int ib = 0; ib < nr; ib += kWave) {
-            const int i = ib + lane;
-            if (i < nr) {
+        // 1. probe: lanes over the peaks of both references, exact candidates
+        //    into the pool
+        for (int ib = 0; ib < total; ib += kWave) {
+            const int e = ib + lane;
+            if (e < total) {
+                const int h = e < nr_a ? 0 : 1;
+                const int i = h ? e - nr_a : e;
+                const int64_t roff = h ? roff_b : roff_a;
                 const float ri = R.mz[roff + i];
                 const float x0 = ri - a.tolc, x1 = ri + a.tolc;
                 auto emit = [&](uint32_t who, float qv) {
@@ -339,13 +469,14 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                     const double mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qp);
                     const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
                     if (!exact_window(mr, mq, a.tol, a.shift)) return;
-                    if (atomicAdd(&cnt[ql], 1) >= kMidCap) return;  // heavy anyway
+                    const int sl = h * kTileQueries + ql;
+                    if (atomicAdd(&cnt[sl], 1) >= kMidCap) return;  // heavy anyway
                     const int pos = atomicAdd(pool_n, 1);
                     if (pos >= kPool) return;
                     pv[pos] = pair_value(R, roff + i, Q, qp);
                     pij[pos] = pack_ij(i, j);
-                    pq[pos] = (unsigned char)ql;
-                    pnext[pos] = atomicExch(&head[ql], pos);
+                    pq[pos] = (unsigned char)sl;
+                    pnext[pos] = atomicExch(&head[sl], pos);
                 };
                 if (in_lds) {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return smz_l[p]; },
@@ -360,14 +491,14 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         }
         wave_sync();
 
-        // 2. settle each query of the tile
+        // 2. settle: lane = (half, query lane)
         const int used = *pool_n;
         const bool over = used > kPool;
-        const int n = lane < kTileQueries ? cnt[lane] : 0;
-        const int64_t q = q0 + lane;
-        const bool valid = lane < kTileQueries && q < a.nQ;
-        const bool complete = !(over && n > 0);  // every candidate of this query is pooled
-        // 2a. up to K candidates: walk the query's list into registers
+        const int n = cnt[lane];
+        const int64_t r = half ? rb2 : ra;
+        const bool valid = q_mine < a.nQ && (half == 0 || has_b);
+        const bool complete = !(over && n > 0);  // every candidate of this pair is pooled
+        // 2a. up to K candidates: walk the pair's list into registers
         const bool small = valid && complete && n <= K;
         int bound = small ? n : 0;  // wave-wide max of the small counts
         for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -384,26 +515,26 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
             }
         }
         if (small) {
-            const int64_t o = r * a.ld + q;
+            const int64_t o = r * a.ld + q_mine;
             double sc = 0.0;
             int m = 0;
             c.greedy(sc, m, bound);
-            a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[lane])) : 0.0f;
+            a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[ql_mine])) : 0.0f;
             a.matches[o] = m;
             a.overflow[o] = (uint8_t)(n >= a.match_limit);
         }
-        // 2b. K < n <= 64: the whole wave settles the query from registers
+        // 2b. K < n <= 64: the whole wave settles the pair from registers
         unsigned long long mid = __ballot(valid && complete && n > K && n <= kMidCap);
         while (mid) {
-            const int ql = __builtin_ctzll(mid);
+            const int sl = __builtin_ctzll(mid);
             mid &= mid - 1ull;
             int have = 0;
             const int lim = used < kPool ? used : kPool;
-            for (int base = 0; base < lim; base += kWave) {  // compact the query's entries
+            for (int base = 0; base < lim; base += kWave) {  // compact the pair's entries
                 const int e = base + lane;
-                const bool mine = e < lim && pq[e] == ql;
+                const bool mine = e < lim && pq[e] == sl;
                 const unsigned long long mk = __ballot(mine);
-                // at most kMidCap entries belong to ql, so every slot is < 64
+                // at most kMidCap entries belong to sl, so every slot is < 64
                 if (mine) midx[have + __popcll(mk & ((1ull << lane) - 1ull))] = e;
                 have += __popcll(mk);
             }
@@ -431,28 +562,27 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                 mk = __ballot(alive);
             }
             if (lane == 0) {
-                const int64_t o = r * a.ld + q0 + ql;
-                const int nn = cnt[ql];
-                a.score[o] = (float)(sc / (R.norm[r] * qnorm_l[ql]));
+                const int64_t rs = (sl >> 5) ? rb2 : ra;
+                const int qls = sl & 31;
+                const int64_t o = rs * a.ld + q0 + qls;
+                a.score[o] = (float)(sc / (R.norm[rs] * qnorm_l[qls]));
                 a.matches[o] = m;
-                a.overflow[o] = (uint8_t)(nn >= a.match_limit);
+                a.overflow[o] = (uint8_t)(cnt[sl] >= a.match_limit);
             }
         }
         // 2c. the rest go to the heavy kernel
-        if (valid && (n > kMidCap || (!complete))) {
-            const int64_t o = r * a.ld + q;
+        if (valid && (n > kMidCap || !complete)) {
+            const int64_t o = r * a.ld + q_mine;
             const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
             if ((int64_t)slot < a.heavy_cap)
-                a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
+                a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q_mine;
             a.score[o] = 0.0f;
             a.matches[o] = -1;
             a.overflow[o] = 0;
         }
         wave_sync();
-        if (lane < kTileQueries) {
-            cnt[lane] = 0;
-            head[lane] = -1;
-        }
+        cnt[lane] = 0;
+        head[lane] = -1;
         if (lane == 0) *pool_n = 0;
         wave_sync();
     }
@@ -1007,6 +1137,10 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
         hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, Q.mz, Q.off,
                            nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw);
         CS_LAUNCH_CHECK("k_tile_meta");
+        hipLaunchKernelGGL(k_tile_index, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.mz,
+                           Q.off, nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo,
+                           L.tile_iw, L.smz, L.swho, L.dir);
+        CS_LAUNCH_CHECK("k_tile_index");
         hipLaunchKernelGGL(k_tile_sort, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.mz, Q.off,
                            nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.smz, L.swho);
         CS_LAUNCH_CHECK("k_tile_sort");
