@@ -19,8 +19,8 @@
 
 namespace cs {
 
-constexpr int kTileQueries = 32;
-constexpr int kTileBuckets = 2048;
+constexpr int kTileQueries = 64;
+constexpr int kTileBuckets = 4096;
 
 CS_HD int tile_bucket(float x, float lo, float inv_w) {
     const float f = (x - lo) * inv_w;

@@ -31,16 +31,16 @@ namespace cs {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kLightWaves = 8;
+constexpr int kLightWaves = 16;
 constexpr int kLightThreads = kWave * kLightWaves;
-constexpr int kLightRefs = 128;       // references one light workgroup walks
-constexpr int kTileLdsCap = 6144;     // sorted tile peaks staged in LDS (larger: read from HBM)
+constexpr int kLightRefs = 256;       // references one light workgroup walks
+constexpr int kTileLdsCap = 15000;    // sorted tile peaks staged in LDS (larger: read from HBM)
 constexpr int kDefaultK = 8;
 constexpr int kDefaultRound = 256;
 constexpr int kHeavyGrid = 4096;
 constexpr int64_t kHeavyQueueCap = int64_t(1) << 20;
 constexpr int kMaxPeaks = 65535;
-constexpr int kIndexStage = 6144;     // tiles up to this many peaks are indexed in LDS
+constexpr int kIndexStage = 12288;    // tiles up to this many peaks are indexed in LDS
 
 struct SpecView {
     const int64_t *off;
@@ -183,16 +183,19 @@ k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, flo
 // then each element's final place inside its bucket is its rank by
 // (value, lane, peak) among the bucket's elements.  Larger tiles are marked
 // in `big` and indexed by k_tile_sort + k_tile_dir.
-__global__ void __launch_bounds__(256)
+constexpr int kIndexThreads = 1024;
+
+__global__ void __launch_bounds__(kIndexThreads)
 k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, float shift32,
              const int32_t *tile_np, const int64_t *tile_pbase, const float *tile_lo,
              const float *tile_iw, float *smz, uint32_t *swho, int32_t *dir) {
+    extern __shared__ __align__(16) unsigned char index_lds[];
     __shared__ int64_t loff[kTileQueries + 1];
-    __shared__ int start[kTileBuckets + 1];
-    __shared__ int cursor[kTileBuckets];
-    __shared__ float bx[kIndexStage];
-    __shared__ uint32_t bw[kIndexStage];
-    __shared__ int part[256];
+    __shared__ int part[kIndexThreads];
+    int *start = reinterpret_cast<int *>(index_lds);     // kTileBuckets + 1
+    int *cursor = start + kTileBuckets + 1;               // kTileBuckets
+    float *bx = reinterpret_cast<float *>(cursor + kTileBuckets);
+    uint32_t *bw = reinterpret_cast<uint32_t *>(bx + kIndexStage);
     const int tid = threadIdx.x;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int np = tile_np[t];
@@ -200,24 +203,24 @@ k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, fl
         const int64_t q0 = t * kTileQueries;
         const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
         if (tid <= nl) loff[tid] = off[q0 + tid];
-        for (int b = tid; b <= kTileBuckets; b += 256) start[b] = 0;
+        for (int b = tid; b <= kTileBuckets; b += kIndexThreads) start[b] = 0;
         __syncthreads();
         const float lo = tile_lo[t], iw = tile_iw[t];
         const int64_t g0 = loff[0];
-        for (int e = tid; e < np; e += 256)
+        for (int e = tid; e < np; e += kIndexThreads)
             atomicAdd(&start[tile_bucket(mz[g0 + e] + shift32, lo, iw)], 1);
         __syncthreads();
         // exclusive scan of start[0..kTileBuckets) (8 buckets per thread)
-        int loc[kTileBuckets / 256];
+        int loc[kTileBuckets / kIndexThreads];
         int sum = 0;
 #pragma unroll
-        for (int u = 0; u < kTileBuckets / 256; ++u) {
-            loc[u] = start[tid * (kTileBuckets / 256) + u];
+        for (int u = 0; u < kTileBuckets / kIndexThreads; ++u) {
+            loc[u] = start[tid * (kTileBuckets / kIndexThreads) + u];
             sum += loc[u];
         }
         part[tid] = sum;
         __syncthreads();
-        for (int o = 1; o < 256; o <<= 1) {
+        for (int o = 1; o < kIndexThreads; o <<= 1) {
             const int v = tid >= o ? part[tid - o] : 0;
             __syncthreads();
             part[tid] += v;
@@ -225,8 +228,8 @@ k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, fl
         }
         int run = tid ? part[tid - 1] : 0;
 #pragma unroll
-        for (int u = 0; u < kTileBuckets / 256; ++u) {
-            const int b = tid * (kTileBuckets / 256) + u;
+        for (int u = 0; u < kTileBuckets / kIndexThreads; ++u) {
+            const int b = tid * (kTileBuckets / kIndexThreads) + u;
             start[b] = run;
             cursor[b] = run;
             run += loc[u];
@@ -234,8 +237,8 @@ k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, fl
         if (tid == 0) start[kTileBuckets] = np;
         __syncthreads();
         int32_t *d = dir + t * (kTileBuckets + 1);
-        for (int b = tid; b <= kTileBuckets; b += 256) d[b] = start[b];
-        for (int e = tid; e < np; e += 256) {  // scatter into buckets
+        for (int b = tid; b <= kTileBuckets; b += kIndexThreads) d[b] = start[b];
+        for (int e = tid; e < np; e += kIndexThreads) {  // scatter into buckets
             const int64_t g = g0 + e;
             int ql = 0;
             while (ql + 1 < nl && loff[ql + 1] <= g) ++ql;
@@ -246,7 +249,7 @@ k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, fl
         }
         __syncthreads();
         const int64_t pb = tile_pbase[t];
-        for (int e = tid; e < np; e += 256) {  // rank inside the bucket
+        for (int e = tid; e < np; e += kIndexThreads) {  // rank inside the bucket
             const float x = bx[e];
             const uint32_t w = bw[e];
             const int b = tile_bucket(x, lo, iw);
@@ -309,6 +312,15 @@ struct LightArgs {
     unsigned long long *counters;
 };
 
+// Value of lane f (wave-uniform f) without an LDS round trip.
+__device__ __forceinline__ int lane_int(int x, int f) { return __builtin_amdgcn_readlane(x, f); }
+__device__ __forceinline__ double lane_double(double x, int f) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, f);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), f);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Orders LDS accesses of the lanes of one wave.  LDS instructions of a wave
 // execute in issue order, so this only has to stop the compiler from moving
 // accesses across it (no s_waitcnt on outstanding global stores).
@@ -321,15 +333,19 @@ __device__ __forceinline__ void wave_sync() {
 // reference against the tile, linked per query (LIFO; order is irrelevant,
 // the settle step sorts).
 constexpr int kPool = 256;
-constexpr int kMidCap = 64;   // candidates a wave settles for one query in registers
-constexpr size_t kWaveLds = (size_t)kPool * (8 + 4 + 4 + 1) + 2 * kTileQueries * 8 + 16 + kWave * 4;
+constexpr int kMidCap = 64;   // candidates a wave settles for one pair in registers
+constexpr size_t kWaveLds =
+    ((size_t)kPool * (8 + 4 + 4 + 1) + (size_t)kTileQueries * 4 * 2 + 16 + kWave * 4 + 15) / 16 * 16;
+constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
 
 // Bytes of dynamic LDS of k_light for a tile capacity `cap`.
 constexpr size_t light_lds_bytes(int cap) {
-    return (size_t)kTileQueries * (8 + 8)                         // qoff, qnorm
-           + (size_t)kLightWaves * ((kWaveLds + 15) / 16 * 16)     // pools
-           + (size_t)(cap + 1) * 8                                 // smz + swho
-           + (size_t)(kTileBuckets + 1) * 2;                       // dir (u16)
+    return (size_t)kTileQueries * (8 + 8)          // qoff, qnorm
+           + (size_t)kLightWaves * kWaveLds         // pools
+           + (size_t)(cap + 1) * 4                  // sorted values (f32)
+           + (size_t)kDirPad * 2                    // directory (u16)
+           + (size_t)(cap + 1) * 2                  // peak index (u16)
+           + (size_t)(cap + 1 + 15) / 16 * 16;      // query lane (u8)
 }
 
 // Descending bitonic sort of one key per lane across the wave (64 lanes).
@@ -354,12 +370,12 @@ __device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
     }
 }
 
-// A wave takes two references at a time: their peaks form one probe stream
-// (fewer idle lanes in the last round), and in the settle step lanes 0-31
-// settle the first reference's pairs with the tile's 32 queries and lanes
-// 32-63 the second's.  Pool slots are keyed by s = half * 32 + query lane.
-constexpr int kSlots = 2 * kTileQueries;
-
+// One workgroup = one 64-query tile x kLightRefs references.  The tile's peak
+// index sits in LDS (shared by 16 waves); wave w settles references
+// w, w + 16, ... one at a time: lanes over the reference's peaks probe the
+// index, exact candidates go to the wave's pool, then lane l settles the pair
+// (reference, query l).  The next reference's first 64 peaks are loaded while
+// the current one is settled.
 template <int K>
 __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     extern __shared__ __align__(16) unsigned char light_lds[];
@@ -369,20 +385,20 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     // LDS layout (see light_lds_bytes)
     int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
     double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
-    unsigned char *wbase = reinterpret_cast<unsigned char *>(qnorm_l + kTileQueries) +
-                           wave * ((kWaveLds + 15) / 16 * 16);
+    unsigned char *pools = reinterpret_cast<unsigned char *>(qnorm_l + kTileQueries);
+    unsigned char *wbase = pools + wave * kWaveLds;
     double *pv = reinterpret_cast<double *>(wbase);
     uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
     int *pnext = reinterpret_cast<int *>(pij + kPool);
     int *cnt = pnext + kPool;
-    int *head = cnt + kSlots;
-    int *pool_n = head + kSlots;
+    int *head = cnt + kTileQueries;
+    int *pool_n = head + kTileQueries;
     int *midx = pool_n + 4;
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
-    float *smz_l = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(qnorm_l + kTileQueries) +
-                                             kLightWaves * ((kWaveLds + 15) / 16 * 16));
-    uint32_t *swho_l = reinterpret_cast<uint32_t *>(smz_l + cap + 1);
-    uint16_t *dir_l = reinterpret_cast<uint16_t *>(swho_l + cap + 1);
+    float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * kWaveLds);
+    uint16_t *dir_l = reinterpret_cast<uint16_t *>(smz_l + cap + 1);
+    uint16_t *wj_l = dir_l + kDirPad;
+    unsigned char *wl_l = reinterpret_cast<unsigned char *>(wj_l + cap + 1);
 
     const int64_t rb = blockIdx.x, t = blockIdx.y;
     const int np = a.tile_np[t];
@@ -393,7 +409,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     const int32_t *gdir = a.dir + t * (kTileBuckets + 1);
     const bool in_lds = np <= cap;
     const int64_t q0 = t * kTileQueries;
-    if (in_lds) {  // four independent loads in flight per thread and array
+    if (in_lds) {  // several independent loads in flight per thread
         for (int e = tid; e <= np; e += 4 * kLightThreads) {
             float x[4];
             uint32_t w[4];
@@ -408,18 +424,19 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                 const int f = e + u * kLightThreads;
                 if (f <= np) {
                     smz_l[f] = x[u];
-                    swho_l[f] = w[u];
+                    wj_l[f] = (uint16_t)(w[u] & 0xffffu);
+                    wl_l[f] = (unsigned char)(w[u] >> 16);
                 }
             }
         }
-        int32_t d[5];
+        int32_t d[(kTileBuckets + 1 + kLightThreads - 1) / kLightThreads];
 #pragma unroll
-        for (int u = 0; u < 5; ++u) {
+        for (int u = 0; u < (kTileBuckets + 1 + kLightThreads - 1) / kLightThreads; ++u) {
             const int b = tid + u * kLightThreads;
             d[u] = b <= kTileBuckets ? gdir[b] : 0;
         }
 #pragma unroll
-        for (int u = 0; u < 5; ++u) {
+        for (int u = 0; u < (kTileBuckets + 1 + kLightThreads - 1) / kLightThreads; ++u) {
             const int b = tid + u * kLightThreads;
             if (b <= kTileBuckets) dir_l[b] = (uint16_t)d[u];
         }
@@ -429,74 +446,86 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         qoff_l[tid] = a.Q.off[q];
         qnorm_l[tid] = a.Q.norm[q];
     }
-    if (tid == 0) next_ref = 0;
     cnt[lane] = 0;
     head[lane] = -1;
     if (lane == 0) *pool_n = 0;
+    if (tid == 0) next_ref = 0;
     __syncthreads();
 
     const int64_t r0 = rb * kLightRefs;
-    const int64_t r1 = r0 + kLightRefs < a.nR ? r0 + kLightRefs : a.nR;
+    const int nrefs = (int)(r0 + kLightRefs < a.nR ? kLightRefs : a.nR - r0);
     const SpecView &R = a.R, &Q = a.Q;
-    const int half = lane >> 5, ql_mine = lane & 31;
-    const int64_t q_mine = q0 + ql_mine;
-    for (;;) {
-        int rr = 0;
-        if (lane == 0) rr = atomicAdd(&next_ref, 2);
-        rr = __shfl(rr, 0);
-        const int64_t ra = r0 + rr;
-        if (ra >= r1) break;
-        const bool has_b = ra + 1 < r1;
-        const int64_t rb2 = has_b ? ra + 1 : ra;
-        const int64_t roff_a = R.off[ra], roff_b = R.off[rb2];
-        const int nr_a = (int)(R.off[ra + 1] - roff_a);
-        const int nr_b = has_b ? (int)(R.off[rb2 + 1] - roff_b) : 0;
-        const int total = nr_a + nr_b;
+    const int64_t q = q0 + lane;
+    const bool valid = q < a.nQ;
 
-        // 1. probe: lanes over the peaks of both references, exact candidates
-        //    into the pool
-        for (int ib = 0; ib < total; ib += kWave) {
-            const int e = ib + lane;
-            if (e < total) {
-                const int h = e < nr_a ? 0 : 1;
-                const int i = h ? e - nr_a : e;
-                const int64_t roff = h ? roff_b : roff_a;
-                const float ri = R.mz[roff + i];
+    // references are taken from a workgroup counter one step ahead, so the
+    // next one's first peaks load while the current one is settled
+    auto grab = [&]() {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(&next_ref, 1);
+        return __shfl(v, 0);
+    };
+    int rr = grab();
+    int64_t roff_n = 0;
+    int nr_n = 0;
+    float ri_n = 0.0f;
+    if (rr < nrefs) {
+        roff_n = R.off[r0 + rr];
+        nr_n = (int)(R.off[r0 + rr + 1] - roff_n);
+        ri_n = lane < nr_n ? R.mz[roff_n + lane] : 0.0f;
+    }
+    while (rr < nrefs) {
+        const int64_t r = r0 + rr;
+        const int64_t roff = roff_n;
+        const int nr = nr_n;
+
+        // 1. probe: lanes over the reference's peaks, exact candidates into the pool
+        for (int ib = 0; ib < nr; ib += kWave) {
+            const int i = ib + lane;
+            const float ri = ib == 0 ? ri_n : (i < nr ? R.mz[roff + i] : 0.0f);
+            if (i < nr) {
                 const float x0 = ri - a.tolc, x1 = ri + a.tolc;
-                auto emit = [&](uint32_t who, float qv) {
-                    const int ql = (int)(who >> 16), j = (int)(who & 0xffffu);
+                auto emit = [&](int ql, int j, float qv) {
                     const int64_t qp = qoff_l[ql] + j;
                     const double mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qp);
                     const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
                     if (!exact_window(mr, mq, a.tol, a.shift)) return;
-                    const int sl = h * kTileQueries + ql;
-                    if (atomicAdd(&cnt[sl], 1) >= kMidCap) return;  // heavy anyway
+                    if (atomicAdd(&cnt[ql], 1) >= kMidCap) return;  // heavy anyway
                     const int pos = atomicAdd(pool_n, 1);
                     if (pos >= kPool) return;
                     pv[pos] = pair_value(R, roff + i, Q, qp);
                     pij[pos] = pack_ij(i, j);
-                    pq[pos] = (unsigned char)sl;
-                    pnext[pos] = atomicExch(&head[sl], pos);
+                    pq[pos] = (unsigned char)ql;
+                    pnext[pos] = atomicExch(&head[ql], pos);
                 };
                 if (in_lds) {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return smz_l[p]; },
                                [&](int b) { return (int)dir_l[b]; },
-                               [&](int p) { emit(swho_l[p], smz_l[p]); });
+                               [&](int p) { emit((int)wl_l[p], (int)wj_l[p], smz_l[p]); });
                 } else {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return gsmz[p]; },
                                [&](int b) { return (int)gdir[b]; },
-                               [&](int p) { emit(gswho[p], gsmz[p]); });
+                               [&](int p) {
+                                   const uint32_t w = gswho[p];
+                                   emit((int)(w >> 16), (int)(w & 0xffffu), gsmz[p]);
+                               });
                 }
             }
         }
         wave_sync();
 
-        // 2. settle: lane = (half, query lane)
+        // prefetch the next reference while this one is settled
+        const int rn = grab();
+        if (rn < nrefs) {
+            roff_n = R.off[r0 + rn];
+            nr_n = (int)(R.off[r0 + rn + 1] - roff_n);
+            ri_n = lane < nr_n ? R.mz[roff_n + lane] : 0.0f;
+        }
+
+        // 2. settle: lane l = pair (r, query l)
         const int used = *pool_n;
         const bool over = used > kPool;
         const int n = cnt[lane];
-        const int64_t r = half ? rb2 : ra;
-        const bool valid = q_mine < a.nQ && (half == 0 || has_b);
         const bool complete = !(over && n > 0);  // every candidate of this pair is pooled
         // 2a. up to K candidates: walk the pair's list into registers
         const bool small = valid && complete && n <= K;
@@ -515,11 +544,11 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
             }
         }
         if (small) {
-            const int64_t o = r * a.ld + q_mine;
+            const int64_t o = r * a.ld + q;
             double sc = 0.0;
             int m = 0;
             c.greedy(sc, m, bound);
-            a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[ql_mine])) : 0.0f;
+            a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[lane])) : 0.0f;
             a.matches[o] = m;
             a.overflow[o] = (uint8_t)(n >= a.match_limit);
         }
@@ -555,27 +584,25 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
             unsigned long long mk = __ballot(alive);
             while (mk) {
                 const int f = __builtin_ctzll(mk);
-                const int fi = __shfl(vi, f), fj = __shfl(vj, f);
-                sc += __shfl(v, f);
+                const int fi = lane_int(vi, f), fj = lane_int(vj, f);
+                sc += lane_double(v, f);
                 ++m;
                 alive = alive && lane != f && vi != fi && vj != fj;
                 mk = __ballot(alive);
             }
             if (lane == 0) {
-                const int64_t rs = (sl >> 5) ? rb2 : ra;
-                const int qls = sl & 31;
-                const int64_t o = rs * a.ld + q0 + qls;
-                a.score[o] = (float)(sc / (R.norm[rs] * qnorm_l[qls]));
+                const int64_t o = r * a.ld + q0 + sl;
+                a.score[o] = (float)(sc / (R.norm[r] * qnorm_l[sl]));
                 a.matches[o] = m;
                 a.overflow[o] = (uint8_t)(cnt[sl] >= a.match_limit);
             }
         }
         // 2c. the rest go to the heavy kernel
         if (valid && (n > kMidCap || !complete)) {
-            const int64_t o = r * a.ld + q_mine;
+            const int64_t o = r * a.ld + q;
             const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
             if ((int64_t)slot < a.heavy_cap)
-                a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q_mine;
+                a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
             a.score[o] = 0.0f;
             a.matches[o] = -1;
             a.overflow[o] = 0;
@@ -585,6 +612,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         head[lane] = -1;
         if (lane == 0) *pool_n = 0;
         wave_sync();
+        rr = rn;
     }
 }
 
@@ -672,8 +700,8 @@ __device__ void greedy_sorted(const double *bv, const uint32_t *bij, int n, uint
         unsigned long long mask = __ballot(alive);
         while (mask) {
             const int f = __builtin_ctzll(mask);
-            const int fi = __shfl(i, f), fj = __shfl(j, f);
-            const double fv = __shfl(v, f);
+            const int fi = lane_int(i, f), fj = lane_int(j, f);
+            const double fv = lane_double(v, f);
             s += fv;
             ++m;
             if (lane == 0) {
@@ -1047,13 +1075,9 @@ int light_capacity(int K) {
             v = 65536;
         max_lds = v;
     }
-    const size_t fixed = light_lds_bytes(0) + 64;  // + static next_ref and slack
+    const size_t fixed = light_lds_bytes(0) + 64;  // + alignment slack
     const size_t room = (size_t)max_lds > fixed ? (size_t)max_lds - fixed : 0;
-    int cap = (int)(room / 8) - 1;
-    // two workgroups per CU when the CU has 160 KiB
-    const size_t two = (size_t)80 * 1024 > fixed ? (size_t)80 * 1024 - fixed : 0;
-    const int cap2 = (int)(two / 8) - 1;
-    cap = cap < cap2 ? cap : cap2;
+    const int cap = (int)(room / 7) - 1;  // 7 bytes per staged peak
     return cap < kTileLdsCap ? (cap > 0 ? cap : 0) : kTileLdsCap;
 }
 
@@ -1137,7 +1161,18 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
         hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, Q.mz, Q.off,
                            nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw);
         CS_LAUNCH_CHECK("k_tile_meta");
-        hipLaunchKernelGGL(k_tile_index, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.mz,
+        const size_t ilds = sizeof(int) * (2 * kTileBuckets + 1) + 8 * (size_t)kIndexStage;
+        static bool index_attr = false;
+        if (!index_attr) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_index),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)ilds);
+            if (e != hipSuccess)
+                return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+            index_attr = true;
+        }
+        hipLaunchKernelGGL(k_tile_index, dim3(blocks(L.ntiles, 1)), dim3(kIndexThreads), ilds, hs,
+                           Q.mz,
                            Q.off, nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo,
                            L.tile_iw, L.smz, L.swho, L.dir);
         CS_LAUNCH_CHECK("k_tile_index");
