@@ -333,7 +333,8 @@ __device__ __forceinline__ void wave_sync() {
 // reference against the tile, linked per query (LIFO; order is irrelevant,
 // the settle step sorts).
 constexpr int kPool = 256;
-constexpr int kMidCap = 64;   // candidates a wave settles for one pair in registers
+constexpr int kMidCap = 64;
+constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first   // candidates a wave settles for one pair in registers
 constexpr size_t kWaveLds =
     ((size_t)kPool * (8 + 4 + 4 + 1) + (size_t)kTileQueries * 4 * 2 + 16 + kWave * 4 + 15) / 16 * 16;
 constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
@@ -600,9 +601,14 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         // 2c. the rest go to the heavy kernel
         if (valid && (n > kMidCap || !complete)) {
             const int64_t o = r * a.ld + q;
-            const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
-            if ((int64_t)slot < a.heavy_cap)
-                a.heavy_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
+            const unsigned long long e = ((unsigned long long)r << 32) | (unsigned int)q;
+            if (n > kBigPair) {  // from the top, taken first
+                const unsigned long long slot = atomicAdd(&a.counters[4], 1ull);
+                if ((int64_t)slot < a.heavy_cap) a.heavy_queue[a.heavy_cap - 1 - slot] = e;
+            } else {
+                const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
+                if ((int64_t)slot < a.heavy_cap) a.heavy_queue[slot] = e;
+            }
             a.score[o] = 0.0f;
             a.matches[o] = -1;
             a.overflow[o] = 0;
@@ -761,27 +767,57 @@ __device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, unsigned ch
 
     auto rmz = [&](int i) { return a.stage ? r_mz[i] : R.mz[roff + i]; };
     auto qms = [&](int j) { return a.stage ? q_ms[j] : Q.mz[qoff + j] + a.shift32; };
-    for (int i = tid; i < nr; i += nt) {
-        const float ri = rmz(i);
-        int lo = 0, hi = nq;  // first query peak not above the window
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (ri - qms(mid) > a.tolc) lo = mid + 1;
-            else hi = mid;
+    const int lane = tid & (kWave - 1);
+    for (int ib = tid - lane; ib < nr; ib += nt) {  // wave-uniform rounds
+        const int i = ib + lane;
+        bool act = i < nr;
+        const float ri = act ? rmz(i) : 0.0f;
+        int j = nq;
+        if (act) {  // first query peak not above the window
+            int lo = 0, hi = nq;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ri - qms(mid) > a.tolc) lo = mid + 1;
+                else hi = mid;
+            }
+            j = lo;
         }
-        const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
-        const double wr = a.stage ? r_w[i] : (R.mzp ? R.mzp[roff + i] : 1.0) * R.inp[roff + i];
-        for (int j = lo; j < nq; ++j) {
-            const float qj = qms(j);
-            if (ri - qj < -a.tolc) break;
-            const double mq = a.exact_from_stage ? (double)qj : exact_mz(Q, qoff + j);
-            if (!exact_window(mr, mq, a.tol, a.shift)) continue;
-            double v = wr * (Q.mzp ? Q.mzp[qoff + j] : 1.0);
-            v = v * (a.stage ? q_d[j] : Q.inp[qoff + j]);
-            const int pos = atomicAdd(bn_s, 1);
-            if (pos < kHeavyBuf) {
-                bv[pos] = v;
-                bij[pos] = pack_ij(i, j);
+        act = act && j < nq;
+        const double mr = act ? (R.mzx ? R.mzx[roff + i] : (double)ri) : 0.0;
+        const double wr =
+            act ? (a.stage ? r_w[i] : (R.mzp ? R.mzp[roff + i] : 1.0) * R.inp[roff + i]) : 0.0;
+        // lockstep over the lanes' windows: one candidate per lane and step,
+        // one LDS atomic per wave and step
+        while (__ballot(act)) {
+            bool push = false;
+            double v = 0.0;
+            uint32_t ij = 0u;
+            if (act) {
+                const float qj = qms(j);
+                if (ri - qj < -a.tolc) {
+                    act = false;
+                } else {
+                    const double mq = a.exact_from_stage ? (double)qj : exact_mz(Q, qoff + j);
+                    if (exact_window(mr, mq, a.tol, a.shift)) {
+                        v = wr * (Q.mzp ? Q.mzp[qoff + j] : 1.0);
+                        v = v * (a.stage ? q_d[j] : Q.inp[qoff + j]);
+                        ij = pack_ij(i, j);
+                        push = true;
+                    }
+                    ++j;
+                    act = j < nq;
+                }
+            }
+            const unsigned long long mask = __ballot(push);
+            int base = 0;
+            if (lane == 0 && mask) base = atomicAdd(bn_s, __popcll(mask));
+            base = __shfl(base, 0);
+            if (push) {
+                const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+                if (pos < kHeavyBuf) {
+                    bv[pos] = v;
+                    bij[pos] = ij;
+                }
             }
         }
     }
@@ -816,17 +852,20 @@ __global__ void __launch_bounds__(kHeavyThreads) k_heavy(HeavyArgs a) {
     __shared__ unsigned long long item_s;
     __shared__ int nlist;
     __shared__ long long list[kHeavyThreads];
-    const unsigned long long count = a.counters[0];
-    if (count == 0) return;
+    // counters[4] counts the big pairs, queued from the top of the queue
+    // downwards; they are taken first so they do not form the tail
+    const unsigned long long count = a.counters[0], big = a.counters[4];
+    if (count + big == 0) return;
     const int tid = threadIdx.x;
-    if ((int64_t)count <= a.heavy_cap) {
+    if ((int64_t)(count + big) <= a.heavy_cap) {
         for (;;) {
             if (tid == 0) item_s = atomicAdd(&a.counters[1], 1ull);
             __syncthreads();
             const unsigned long long item = item_s;
             __syncthreads();
-            if (item >= count) break;
-            const unsigned long long rq = a.heavy_queue[item];
+            if (item >= count + big) break;
+            const unsigned long long rq =
+                item < big ? a.heavy_queue[a.heavy_cap - 1 - item] : a.heavy_queue[item - big];
             heavy_pair(a, (int64_t)(rq >> 32), (int64_t)(rq & 0xffffffffull), heavy_lds, &bn_s);
         }
     } else {  // queue overflowed: find the -1 sentinels
@@ -1040,7 +1079,7 @@ Layout layout(void *base, const cs_spectra &R, const cs_spectra &Q) {
     L.smz = c.take<float>(Q.n_peaks + T);
     L.swho = c.take<uint32_t>(Q.n_peaks + T);
     L.dir = c.take<int32_t>(T * (kTileBuckets + 1));
-    L.counters = c.take<unsigned long long>(4);
+    L.counters = c.take<unsigned long long>(8);
     L.heavy_queue = c.take<unsigned long long>(L.heavy_cap);
     L.giant_queue = c.take<unsigned long long>(L.heavy_cap);
     L.bytes = align_up(c.used, 256);
@@ -1191,7 +1230,7 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     const int64_t ml = params->match_limit;
     if (stages & CS_STAGE_LIGHT) {
         // the heavy-pair queue of this call starts empty
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)4);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)8);
         CS_LAUNCH_CHECK("k_fill_u64");
         LightArgs la{};
         la.R = R;
