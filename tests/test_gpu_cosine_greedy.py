@@ -344,3 +344,27 @@ def test_giant_pairs(gpu, oracle, T):
     want = oracle_csr(oracle, p, p, tolerance=1.0, match_limit=5000)
     assert want["n_candidates"].max() > 4096
     assert_parity(res, want, 5000)
+
+
+def test_heavy_queue_overflow(gpu):
+    """More heavy pairs than the 2^20-entry queue holds: 1040 spectra whose 70
+    peaks all sit at the same m/z (1 Da apart, tol 0.1), so every pair has
+    exactly the 70 diagonal candidates (above the 64 the lane kernel settles).
+    The overflowing queue is abandoned for a scan of the matches plane; the
+    closed form is matches = 70 and score = cosine of the intensity vectors."""
+    from cudams.spectra import PackedSpectra
+
+    n, k = 1040, 70
+    assert n * n > (1 << 20)
+    rng = np.random.default_rng(21)
+    mz = 100.0 + np.arange(k, dtype=np.float64)
+    its = rng.random((n, k)).astype(np.float32).astype(np.float64) + 0.05
+    p = PackedSpectra.from_arrays([mz] * n, list(its))
+    say(f"heavy queue overflow {n}x{n}")
+    res = device_run(p, p, tol=0.1)
+    assert np.all(res["matches"] == k)
+    w = p.intensities.reshape(n, k)
+    norm = np.sqrt((w * w).sum(axis=1))
+    want = (w @ w.T) / np.outer(norm, norm)
+    np.testing.assert_allclose(res["score"], want, rtol=0, atol=SCORE_ATOL)
+    assert np.all(res["overflow"] == 0)
