@@ -96,9 +96,14 @@ typedef struct cs_cosine_params {
 } cs_cosine_params;
 
 enum {
-    CS_STAGE_PREPARE = 1,    /* length-sort the queries into interleaved tiles    */
-    CS_STAGE_LIGHT = 2,      /* one lane per pair, <= light_candidates candidates */
-    CS_STAGE_HEAVY = 4       /* one wave per remaining pair                       */
+    CS_STAGE_PREPARE = 1,    /* index the queries: 64-query tiles whose merged,
+                                sorted peak list and bucket directory live in the
+                                workspace                                         */
+    CS_STAGE_LIGHT = 2,      /* probe every reference against every tile; pairs
+                                with <= 64 candidates are settled here, the rest
+                                are queued                                        */
+    CS_STAGE_HEAVY = 4       /* one workgroup per queued pair (one wave per pair
+                                beyond 4096 candidates)                           */
 };
 
 /* ------------------------------------------------------------------------ */
