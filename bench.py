@@ -138,7 +138,7 @@ def bench_cosine(a) -> dict:
     light_s = stage_ms["light"] / 1e3
     achieved = alg_bytes / light_s / 1e9
     res = {
-        "metric": "CosineGreedy pairs/s at 8k x 8k",
+        "metric": "CosineGreedy pairs/s at 8k x 8k; p50/p99 search latency served vs plain",
         "value": pairs / (ms / 1e3),
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -229,6 +229,94 @@ def bench_copy(a) -> dict:
             "roofline": None, "cpu_baseline": None}
 
 
+def bench_serving(a) -> dict:
+    """Served library search: 500,000 library spectra (lognormal peaks, no
+    clusters, seed 7) resident on the GPU and indexed once; requests of 1-256
+    spectra (log-uniform) from a 4096-spectrum query pool (seed 8) arrive as a
+    Poisson stream; each asks for the top-10 library hits per spectrum at
+    tolerance 0.01.  Served = continuous batching (3 slots sharing a 6 GB
+    output budget); plain = one search + top-k call per request, in arrival
+    order, on the same index.  Latency = completion - scheduled arrival."""
+    import numpy as np
+
+    from cudams import synthetic
+    from cudams.serving import LibraryIndex, pack_request, run_load
+    from cudams.spectra import PackedSpectra
+
+    n_lib = a.serve_library
+    t0 = time.perf_counter()
+    lib = synthetic.lognormal_spectra(n_lib, 7, cluster_fraction=0.0)
+    log(f"serving: library {n_lib} spectra, {int(lib.offsets[-1])} peaks "
+        f"({time.perf_counter() - t0:.1f} s)")
+    ix = LibraryIndex(lib, tolerance=0.01)
+    pool = synthetic.lognormal_spectra(4096, 8, cluster_fraction=0.0)
+
+    def request(start, size):
+        off = pool.offsets[start:start + size + 1]
+        return pack_request(PackedSpectra(pool.mz[off[0]:off[-1]],
+                                          pool.intensities[off[0]:off[-1]], off - off[0]))
+
+    def stream(rate, seconds, seed):
+        rng = np.random.default_rng(seed)
+        n = max(1, int(rng.poisson(rate * seconds)))
+        arrivals = np.sort(rng.uniform(0.0, seconds, n)) if rate > 0 else np.zeros(n)
+        sizes = np.clip(np.floor(np.exp(rng.uniform(0.0, np.log(257.0), n))).astype(int), 1, 256)
+        starts = rng.integers(0, pool.n_spectra - 256, n)
+        return [request(s0, sz) for s0, sz in zip(starts, sizes)], arrivals
+
+    # warm both paths (kernel attributes, allocator, threads)
+    reqs, arr = stream(100.0, 0.5, 99)
+    run_load("served", ix, reqs, arr)
+    run_load("plain", ix, reqs, arr)
+    rates = []
+    for rate in (50.0, 200.0, 1000.0):
+        reqs, arr = stream(rate, a.serve_seconds, int(rate))
+        row = {"rate_per_s": rate, "seconds": a.serve_seconds}
+        for mode in ("served", "plain"):
+            r = run_load(mode, ix, reqs, arr)
+            log(f"serving rate {rate}: {mode} p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms "
+                f"{r['requests_per_s']:.0f} req/s")
+            row[mode] = {k: r[k] for k in ("p50_ms", "p99_ms", "mean_ms", "requests_per_s",
+                                           "query_spectra_per_s", "requests")}
+            if mode == "served":
+                row[mode]["mean_batch_spectra"] = r["mean_batch_spectra"]
+        rates.append(row)
+    # saturation: a burst of requests all arriving at once; throughput =
+    # requests / makespan
+    reqs = [request(s0, sz) for s0, sz in zip(
+        np.random.default_rng(6).integers(0, pool.n_spectra - 256, a.serve_burst),
+        np.clip(np.floor(np.exp(np.random.default_rng(7).uniform(0.0, np.log(257.0),
+                                                                 a.serve_burst))).astype(int),
+                1, 256))]
+    arr = np.zeros(len(reqs))
+    sat = {}
+    for mode in ("served", "plain"):
+        r = run_load(mode, ix, reqs, arr)
+        sat[mode] = {"requests_per_s": r["requests_per_s"],
+                     "query_spectra_per_s": r["query_spectra_per_s"], "p99_ms": r["p99_ms"]}
+        log(f"serving burst of {len(reqs)}: {mode} {r['requests_per_s']:.0f} req/s")
+    sat["throughput_ratio"] = sat["served"]["requests_per_s"] / sat["plain"]["requests_per_s"]
+    out = {"library_spectra": n_lib, "library_peaks": int(lib.offsets[-1]), "top_k": 10,
+           "tolerance": 0.01, "request_spectra": "log-uniform 1-256", "rates": rates,
+           "saturation": {"burst_requests": len(reqs), **sat}}
+    if not a.no_cpu_baseline:
+        from oracle import oracle
+
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        csr = (lib.mz, lib.intensities, lib.offsets)
+        cpu = {}
+        for size in (1, 16):
+            off = pool.offsets[:size + 1]
+            q = (pool.mz[:off[-1]], pool.intensities[:off[-1]], off)
+            t1 = time.perf_counter()
+            oracle.cosine_greedy_matrix(q, csr, tolerance=0.01, nthreads=cores)
+            cpu[str(size)] = (time.perf_counter() - t1) * 1e3
+        out["cpu_request_latency_ms"] = {"cores": cores, "kind": "port", **cpu,
+                                         "note": "oracle/cs_oracle.c float64 all pairs of the "
+                                                 "request x library, top-k not included"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,6 +328,11 @@ def main() -> int:
     ap.add_argument("--round-capacity", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the served-search section of the cosine line")
+    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-library", type=int, default=500_000)
+    ap.add_argument("--serve-burst", type=int, default=1500)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); this build has no "
@@ -251,6 +344,8 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     res = {"cosine": bench_cosine, "fingerprint": bench_fingerprint, "copy": bench_copy}[a.workload](a)
+    if a.workload == "cosine" and not a.no_serving:
+        res["serving"] = bench_serving(a)
     print(json.dumps(res), flush=True)
     return 0
 

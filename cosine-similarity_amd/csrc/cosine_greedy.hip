@@ -310,6 +310,7 @@ struct LightArgs {
     unsigned long long *heavy_queue;
     int64_t heavy_cap;
     unsigned long long *counters;
+    int64_t n_ref_blocks;  // grid = n_ref_blocks x tiles, reference block fastest
 };
 
 // Value of lane f (wave-uniform f) without an LDS round trip.
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     uint16_t *wj_l = dir_l + kDirPad;
     unsigned char *wl_l = reinterpret_cast<unsigned char *>(wj_l + cap + 1);
 
-    const int64_t rb = blockIdx.x, t = blockIdx.y;
+    const int64_t rb = (int64_t)blockIdx.x % a.n_ref_blocks, t = (int64_t)blockIdx.x / a.n_ref_blocks;
     const int np = a.tile_np[t];
     const int64_t pb = a.tile_pbase[t];
     const float lo = a.tile_lo[t], iw = a.tile_iw[t];
@@ -1050,28 +1051,29 @@ __global__ void __launch_bounds__(kWave) k_giant(HeavyArgs a) {
 // ---------------------------------------------------------------------------
 
 struct Layout {
+    // query index (written by PREPARE, read-only afterwards)
     int32_t *tile_np;
     int64_t *tile_pbase;
     float *tile_lo, *tile_iw, *smz;
     uint32_t *swho;
     int32_t *dir;
+    // per-call scratch (LIGHT / HEAVY)
     unsigned long long *counters, *heavy_queue, *giant_queue;
     int64_t ntiles, heavy_cap;
-    size_t bytes;
+    size_t index_bytes, scratch_bytes;
 };
 
-Layout layout(void *base, const cs_spectra &R, const cs_spectra &Q) {
+// The query index and the call scratch are carved from two bases.  Their
+// sizes depend on the queries only (the heavy queue has a fixed capacity;
+// pairs beyond it are found by scanning for the -1 sentinel instead), so one
+// prepared index serves any reference set, and calls with their own scratch
+// may share it.
+Layout layout(void *index_base, void *scratch_base, const cs_spectra &Q) {
     Layout L{};
-    Carver c(base);
     L.ntiles = (Q.n_spectra + kTileQueries - 1) / kTileQueries;
-    // fixed capacity: the workspace size depends on the queries only, so one
-    // prepared workspace serves any reference set (pairs beyond the capacity
-    // are found by scanning for the -1 sentinel instead)
-    (void)R;
     L.heavy_cap = kHeavyQueueCap;
     const int64_t T = L.ntiles > 0 ? L.ntiles : 1;
-    // query-only buffers first: their offsets do not depend on the references,
-    // so a PREPARE done once serves later calls with other references
+    Carver c(index_base);
     L.tile_np = c.take<int32_t>(T);
     L.tile_pbase = c.take<int64_t>(T);
     L.tile_lo = c.take<float>(T);
@@ -1079,10 +1081,12 @@ Layout layout(void *base, const cs_spectra &R, const cs_spectra &Q) {
     L.smz = c.take<float>(Q.n_peaks + T);
     L.swho = c.take<uint32_t>(Q.n_peaks + T);
     L.dir = c.take<int32_t>(T * (kTileBuckets + 1));
-    L.counters = c.take<unsigned long long>(8);
-    L.heavy_queue = c.take<unsigned long long>(L.heavy_cap);
-    L.giant_queue = c.take<unsigned long long>(L.heavy_cap);
-    L.bytes = align_up(c.used, 256);
+    L.index_bytes = align_up(c.used, 256);
+    Carver d(scratch_base);
+    L.counters = d.take<unsigned long long>(8);
+    L.heavy_queue = d.take<unsigned long long>(L.heavy_cap);
+    L.giant_queue = d.take<unsigned long long>(L.heavy_cap);
+    L.scratch_bytes = align_up(d.used, 256);
     return L;
 }
 
@@ -1136,52 +1140,16 @@ cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
     return CS_OK;
 }
 
-}  // namespace
-}  // namespace cs
-
-using namespace cs;
-
-extern "C" cs_status cs_cosine_greedy_workspace_size(const cs_spectra *refs,
-                                                     const cs_spectra *queries, size_t *bytes) {
+// One whole-pipeline or staged call with the index and scratch at two bases.
+cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
+                            const cs_cosine_params *params, float *score, int32_t *matches,
+                            uint8_t *overflow, int64_t ld, const Layout &L, void *stream) {
     cs_status st;
-    if ((st = check_spectra(refs, "refs")) != CS_OK) return st;
-    if ((st = check_spectra(queries, "queries")) != CS_OK) return st;
-    if (!bytes) return fail(CS_ERR_INVALID, "bytes is NULL");
-    *bytes = layout(nullptr, *refs, *queries).bytes;
-    return CS_OK;
-}
-
-extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *queries,
-                                      const cs_cosine_params *params, float *score,
-                                      int32_t *matches, uint8_t *overflow, int64_t ld,
-                                      void *workspace, size_t workspace_bytes, void *stream) {
-    cs_status st;
-    if ((st = check_spectra(refs, "refs")) != CS_OK) return st;
-    if ((st = check_spectra(queries, "queries")) != CS_OK) return st;
-    if (!params) return fail(CS_ERR_INVALID, "params is NULL");
     const cs_spectra &Rs = *refs, &Qs = *queries;
     const int64_t nR = Rs.n_spectra, nQ = Qs.n_spectra;
-    if (nR == 0 || nQ == 0) return CS_OK;
-    if (!score || !matches || !overflow) return fail(CS_ERR_INVALID, "output pointer is NULL");
-    if (ld < nQ) return fail(CS_ERR_INVALID, "ld %lld < number of queries %lld", (long long)ld,
-                             (long long)nQ);
     const double tol = params->tolerance, shift = params->shift;
-    if (!(tol >= 0.0) || tol == __builtin_huge_val())
-        return fail(CS_ERR_INVALID, "tolerance must be finite and >= 0");
-    if (!(shift == shift) || shift == __builtin_huge_val() || shift == -__builtin_huge_val())
-        return fail(CS_ERR_INVALID, "shift must be finite");
     int K = params->light_candidates ? params->light_candidates : kDefaultK;
-    if (K != 1 && K != 4 && K != 8 && K != 16)
-        return fail(CS_ERR_INVALID, "light_candidates must be 1, 4, 8 or 16 (got %d)", K);
     const int T = params->round_capacity ? params->round_capacity : kDefaultRound;
-    if (T < 4 || T > 1024 || (T & (T - 1)))
-        return fail(CS_ERR_INVALID, "round_capacity must be a power of two in [4, 1024]");
-
-    const Layout need = layout(nullptr, Rs, Qs);
-    if (!workspace || workspace_bytes < need.bytes)
-        return fail(CS_ERR_WORKSPACE, "workspace of %zu bytes < %zu required", workspace_bytes,
-                    need.bytes);
-    const Layout L = layout(workspace, Rs, Qs);
     hipStream_t hs = static_cast<hipStream_t>(stream);
     const SpecView R = view(Rs), Q = view(Qs);
     const float mzmax = Rs.max_abs_mz > Qs.max_abs_mz ? Rs.max_abs_mz : Qs.max_abs_mz;
@@ -1191,11 +1159,9 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
         int64_t b = (n + per - 1) / per;
         return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
     };
-
     const int stages = params->stages ? params->stages
                                       : (CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY);
-    if (stages & ~(CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY))
-        return fail(CS_ERR_INVALID, "unknown stage bits 0x%x", stages);
+    if (nQ == 0) return CS_OK;
     if (stages & CS_STAGE_PREPARE) {
         hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, Q.mz, Q.off,
                            nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw);
@@ -1211,8 +1177,7 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
             index_attr = true;
         }
         hipLaunchKernelGGL(k_tile_index, dim3(blocks(L.ntiles, 1)), dim3(kIndexThreads), ilds, hs,
-                           Q.mz,
-                           Q.off, nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo,
+                           Q.mz, Q.off, nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo,
                            L.tile_iw, L.smz, L.swho, L.dir);
         CS_LAUNCH_CHECK("k_tile_index");
         hipLaunchKernelGGL(k_tile_sort, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.mz, Q.off,
@@ -1222,10 +1187,11 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
                            L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz, L.dir);
         CS_LAUNCH_CHECK("k_tile_dir");
     }
+    if (nR == 0) return CS_OK;
 
     const int64_t nrb = (nR + kLightRefs - 1) / kLightRefs;
-    if (nrb > INT_MAX || L.ntiles > 65535)
-        return fail(CS_ERR_UNSUPPORTED, "grid too large (%lld ref blocks, %lld query tiles)",
+    if (nrb * L.ntiles > INT_MAX)
+        return fail(CS_ERR_UNSUPPORTED, "grid too large (%lld ref blocks x %lld query tiles)",
                     (long long)nrb, (long long)L.ntiles);
     const int64_t ml = params->match_limit;
     if (stages & CS_STAGE_LIGHT) {
@@ -1256,7 +1222,8 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
         la.heavy_queue = L.heavy_queue;
         la.heavy_cap = L.heavy_cap;
         la.counters = L.counters;
-        const dim3 grid((unsigned)nrb, (unsigned)L.ntiles);
+        la.n_ref_blocks = nrb;
+        const dim3 grid((unsigned)(nrb * L.ntiles));
         switch (K) {
             case 1: st = launch_light<1>(grid, hs, la); break;
             case 4: st = launch_light<4>(grid, hs, la); break;
@@ -1317,4 +1284,103 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
     hipLaunchKernelGGL(k_giant, dim3(kGiantGrid), dim3(kWave), glds, hs, a);
     CS_LAUNCH_CHECK("k_giant");
     return CS_OK;
+}
+
+// Argument checks shared by the entry points (no HIP call).
+cs_status check_call(const cs_spectra *refs, const cs_spectra *queries,
+                     const cs_cosine_params *params, const float *score, const int32_t *matches,
+                     const uint8_t *overflow, int64_t ld) {
+    cs_status st;
+    if ((st = check_spectra(refs, "refs")) != CS_OK) return st;
+    if ((st = check_spectra(queries, "queries")) != CS_OK) return st;
+    if (!params) return fail(CS_ERR_INVALID, "params is NULL");
+    const int stages = params->stages;
+    if (stages & ~(CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY))
+        return fail(CS_ERR_INVALID, "unknown stage bits 0x%x", stages);
+    const bool computes = stages == 0 || (stages & (CS_STAGE_LIGHT | CS_STAGE_HEAVY));
+    if (computes && refs->n_spectra > 0 && queries->n_spectra > 0) {
+        if (!score || !matches || !overflow) return fail(CS_ERR_INVALID, "output pointer is NULL");
+        if (ld < queries->n_spectra)
+            return fail(CS_ERR_INVALID, "ld %lld < number of queries %lld", (long long)ld,
+                        (long long)queries->n_spectra);
+    }
+    const double tol = params->tolerance, shift = params->shift;
+    if (!(tol >= 0.0) || tol == __builtin_huge_val())
+        return fail(CS_ERR_INVALID, "tolerance must be finite and >= 0");
+    if (!(shift == shift) || shift == __builtin_huge_val() || shift == -__builtin_huge_val())
+        return fail(CS_ERR_INVALID, "shift must be finite");
+    const int K = params->light_candidates ? params->light_candidates : kDefaultK;
+    if (K != 1 && K != 4 && K != 8 && K != 16)
+        return fail(CS_ERR_INVALID, "light_candidates must be 1, 4, 8 or 16 (got %d)", K);
+    const int T = params->round_capacity ? params->round_capacity : kDefaultRound;
+    if (T < 4 || T > 1024 || (T & (T - 1)))
+        return fail(CS_ERR_INVALID, "round_capacity must be a power of two in [4, 1024]");
+    return CS_OK;
+}
+
+}  // namespace
+}  // namespace cs
+
+using namespace cs;
+
+extern "C" cs_status cs_cosine_greedy_workspace_size(const cs_spectra *refs,
+                                                     const cs_spectra *queries, size_t *bytes) {
+    cs_status st;
+    if ((st = check_spectra(refs, "refs")) != CS_OK) return st;
+    if ((st = check_spectra(queries, "queries")) != CS_OK) return st;
+    if (!bytes) return fail(CS_ERR_INVALID, "bytes is NULL");
+    const Layout L = layout(nullptr, nullptr, *queries);
+    *bytes = L.index_bytes + L.scratch_bytes;
+    return CS_OK;
+}
+
+extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *queries,
+                                      const cs_cosine_params *params, float *score,
+                                      int32_t *matches, uint8_t *overflow, int64_t ld,
+                                      void *workspace, size_t workspace_bytes, void *stream) {
+    cs_status st;
+    if ((st = check_call(refs, queries, params, score, matches, overflow, ld)) != CS_OK) return st;
+    if (queries->n_spectra == 0) return CS_OK;
+    if (refs->n_spectra == 0 && !(params->stages & CS_STAGE_PREPARE)) return CS_OK;
+    const Layout need = layout(nullptr, nullptr, *queries);
+    const size_t total = need.index_bytes + need.scratch_bytes;
+    if (!workspace || workspace_bytes < total)
+        return fail(CS_ERR_WORKSPACE, "workspace of %zu bytes < %zu required", workspace_bytes,
+                    total);
+    char *base = static_cast<char *>(workspace);
+    const Layout L = layout(base, base + need.index_bytes, *queries);
+    return cosine_greedy_run(refs, queries, params, score, matches, overflow, ld, L, stream);
+}
+
+extern "C" cs_status cs_cosine_greedy_indexed_sizes(const cs_spectra *queries, size_t *index_bytes,
+                                                    size_t *scratch_bytes) {
+    cs_status st;
+    if ((st = check_spectra(queries, "queries")) != CS_OK) return st;
+    if (!index_bytes || !scratch_bytes) return fail(CS_ERR_INVALID, "size pointer is NULL");
+    const Layout L = layout(nullptr, nullptr, *queries);
+    *index_bytes = L.index_bytes;
+    *scratch_bytes = L.scratch_bytes;
+    return CS_OK;
+}
+
+extern "C" cs_status cs_cosine_greedy_indexed(const cs_spectra *refs, const cs_spectra *queries,
+                                              const cs_cosine_params *params, float *score,
+                                              int32_t *matches, uint8_t *overflow, int64_t ld,
+                                              void *index, size_t index_bytes, void *scratch,
+                                              size_t scratch_bytes, void *stream) {
+    cs_status st;
+    if ((st = check_call(refs, queries, params, score, matches, overflow, ld)) != CS_OK) return st;
+    if (queries->n_spectra == 0) return CS_OK;
+    if (refs->n_spectra == 0 && !(params->stages & CS_STAGE_PREPARE)) return CS_OK;
+    const Layout need = layout(nullptr, nullptr, *queries);
+    if (!index || index_bytes < need.index_bytes)
+        return fail(CS_ERR_WORKSPACE, "index of %zu bytes < %zu required", index_bytes,
+                    need.index_bytes);
+    const int stages = params->stages ? params->stages
+                                      : (CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY);
+    if ((stages & (CS_STAGE_LIGHT | CS_STAGE_HEAVY)) && (!scratch || scratch_bytes < need.scratch_bytes))
+        return fail(CS_ERR_WORKSPACE, "scratch of %zu bytes < %zu required", scratch_bytes,
+                    need.scratch_bytes);
+    const Layout L = layout(index, scratch, *queries);
+    return cosine_greedy_run(refs, queries, params, score, matches, overflow, ld, L, stream);
 }

@@ -11,9 +11,12 @@ namespace cs {
 namespace {
 
 constexpr int kTopkThreads = 256;
+constexpr int kTopkChunk = 4096;   // columns one block ranks in the first pass
+constexpr int kTopkLoads = kTopkChunk / kTopkThreads;
 constexpr int kMaxK = 16;
 
-// Ranking of one hit: higher score first, then lower column.  NaN ranks last.
+// Ranking of one hit: higher score first, then lower column.  NaN ranks last;
+// empty slots are (NaN, INT_MAX) and rank after every real hit.
 __device__ __forceinline__ bool hit_before(float sa, int ca, float sb, int cb) {
     const bool na = sa != sa, nb = sb != sb;
     if (na) return nb && ca < cb;
@@ -21,74 +24,178 @@ __device__ __forceinline__ bool hit_before(float sa, int ca, float sb, int cb) {
     return sa > sb || (sa == sb && ca < cb);
 }
 
+// The K best hits one thread has seen, best first, in registers.
+template <int K>
+struct TopList {
+    float s[K];
+    int c[K];
+    __device__ void clear() {
+#pragma unroll
+        for (int x = 0; x < K; ++x) {
+            s[x] = __builtin_nanf("");
+            c[x] = INT_MAX;
+        }
+    }
+    __device__ void insert(float v, int cc) {
+        if (!hit_before(v, cc, s[K - 1], c[K - 1])) return;
+#pragma unroll
+        for (int x = 0; x < K; ++x) {  // bubble the new hit into place
+            const bool take = hit_before(v, cc, s[x], c[x]);
+            const float tv = take ? s[x] : v;
+            const int tc = take ? c[x] : cc;
+            s[x] = take ? v : s[x];
+            c[x] = take ? cc : c[x];
+            v = tv;
+            cc = tc;
+        }
+    }
+    // drop the head (the caller's list held the wave's best hit)
+    __device__ void pop() {
+#pragma unroll
+        for (int x = 0; x + 1 < K; ++x) {
+            s[x] = s[x + 1];
+            c[x] = c[x + 1];
+        }
+        s[K - 1] = __builtin_nanf("");
+        c[K - 1] = INT_MAX;
+    }
+};
+
+// Best head over the wave (every lane gets it).
+__device__ __forceinline__ void wave_best(float &bs, int &bc) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float os = __shfl_xor(bs, off);
+        const int oc = __shfl_xor(bc, off);
+        if (hit_before(os, oc, bs, bc)) {
+            bs = os;
+            bc = oc;
+        }
+    }
+}
+
+// Merges the threads' lists of one block; the block's k best (k <= K) end
+// in fs[0..k) / fc[0..k) (LDS).  Columns are distinct, so exactly one lane
+// owns each real head; empty heads are all alike and pop together.
+template <int K>
+__device__ void block_merge(TopList<K> &L, int k, float *ws, int *wc, float *fs, int *fc) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int x = 0; x < k; ++x) {  // each wave's k best
+        float bs = L.s[0];
+        int bc = L.c[0];
+        wave_best(bs, bc);
+        if (lane == 0) {
+            ws[wave * K + x] = bs;
+            wc[wave * K + x] = bc;
+        }
+        if (L.c[0] == bc) L.pop();
+    }
+    __syncthreads();
+    if (wave == 0) {  // merge the waves' lists, one candidate per lane
+        constexpr int kWaves = kTopkThreads / 64;
+        static_assert(kWaves * K <= 64, "one candidate per lane");
+        const bool mine = lane < kWaves * K && (lane % K) < k;
+        float hs = mine ? ws[lane] : __builtin_nanf("");
+        int hc = mine ? wc[lane] : INT_MAX;
+        for (int x = 0; x < k; ++x) {
+            float bs = hs;
+            int bc = hc;
+            wave_best(bs, bc);
+            if (lane == 0) {
+                fs[x] = bs;
+                fc[x] = bc;
+            }
+            if (hc == bc) {
+                hs = __builtin_nanf("");
+                hc = INT_MAX;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void write_hits(const float *fs, const int *fc, int k, int64_t row,
+                                           int64_t cols, int64_t ld, const int32_t *matches,
+                                           float *top_score, int32_t *top_index,
+                                           int32_t *top_matches) {
+    for (int x = threadIdx.x; x < k; x += blockDim.x) {
+        const int c = fc[x];
+        const bool real = c != INT_MAX && (int64_t)c < cols;
+        top_score[row * k + x] = real ? fs[x] : __builtin_nanf("");
+        top_index[row * k + x] = real ? c : -1;
+        if (top_matches) top_matches[row * k + x] = real ? matches[row * ld + c] : 0;
+    }
+}
+
+// Pass 1: block b ranks columns [chunk * kTopkChunk, ...) of one row, with
+// all of a thread's loads issued before its first comparison.  One chunk per
+// row: the result is final; otherwise the chunk's k best go to the parts.
 template <int K>
 __global__ void __launch_bounds__(kTopkThreads)
-k_topk_rows(const float *__restrict__ score, const int32_t *__restrict__ matches, int64_t rows,
-            int64_t cols, int64_t ld, int k, float *__restrict__ top_score,
-            int32_t *__restrict__ top_index, int32_t *__restrict__ top_matches) {
-    __shared__ float ls[kTopkThreads * K];
-    __shared__ int lc[kTopkThreads * K];
+k_topk_chunks(const float *__restrict__ score, const int32_t *__restrict__ matches, int64_t rows,
+              int64_t cols, int64_t ld, int k, int64_t nchunk, float *__restrict__ part_s,
+              int *__restrict__ part_c, float *__restrict__ top_score,
+              int32_t *__restrict__ top_index, int32_t *__restrict__ top_matches) {
+    __shared__ float ws[kTopkThreads / 64 * K], fs[K];
+    __shared__ int wc[kTopkThreads / 64 * K], fc[K];
     const int tid = threadIdx.x;
-    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+    for (int64_t b = blockIdx.x; b < rows * nchunk; b += gridDim.x) {
+        const int64_t row = b / nchunk, chunk = b - row * nchunk;
+        const int64_t c0 = chunk * kTopkChunk;
         const float *s = score + row * ld;
-        float bs[K];
-        int bc[K];
+        float v[kTopkLoads];
 #pragma unroll
-        for (int x = 0; x < K; ++x) {
-            bs[x] = __builtin_nanf("");
-            bc[x] = INT_MAX;
+        for (int u = 0; u < kTopkLoads; ++u) {
+            const int64_t c = c0 + u * kTopkThreads + tid;
+            v[u] = c < cols ? s[c] : 0.0f;
         }
-        for (int64_t c = tid; c < cols; c += kTopkThreads) {
-            float v = s[c];
-            int cc = (int)c;
-            if (!hit_before(v, cc, bs[K - 1], bc[K - 1])) continue;
+        TopList<K> L;
+        L.clear();
 #pragma unroll
-            for (int x = 0; x < K; ++x) {  // bubble the new hit into place
-                const bool take = hit_before(v, cc, bs[x], bc[x]);
-                const float tv = take ? bs[x] : v;
-                const int tc = take ? bc[x] : cc;
-                bs[x] = take ? v : bs[x];
-                bc[x] = take ? cc : bc[x];
-                v = tv;
-                cc = tc;
+        for (int u = 0; u < kTopkLoads; ++u) {
+            const int64_t c = c0 + u * kTopkThreads + tid;
+            if (c < cols) L.insert(v[u], (int)c);
+        }
+        block_merge<K>(L, k, ws, wc, fs, fc);
+        if (nchunk == 1) {
+            write_hits(fs, fc, k, row, cols, ld, matches, top_score, top_index, top_matches);
+        } else {
+            for (int x = tid; x < k; x += kTopkThreads) {
+                part_s[b * K + x] = fs[x];
+                part_c[b * K + x] = fc[x];
             }
-        }
-#pragma unroll
-        for (int x = 0; x < K; ++x) {
-            ls[x * kTopkThreads + tid] = bs[x];
-            lc[x * kTopkThreads + tid] = bc[x];
-        }
-        __syncthreads();
-        // bitonic sort of the kTopkThreads * K candidates by hit_before
-        const int n = kTopkThreads * K;
-        for (int size = 2; size <= n; size <<= 1) {
-            for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int t = tid; t < n / 2; t += kTopkThreads) {
-                    const int lo = 2 * t - (t & (stride - 1));
-                    const int hi = lo + stride;
-                    const bool up = (lo & size) == 0;
-                    const float a = ls[lo], b = ls[hi];
-                    const int ca = lc[lo], cb = lc[hi];
-                    const bool sw = up ? hit_before(b, cb, a, ca) : hit_before(a, ca, b, cb);
-                    if (sw) {
-                        ls[lo] = b;
-                        ls[hi] = a;
-                        lc[lo] = cb;
-                        lc[hi] = ca;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        for (int x = tid; x < k; x += kTopkThreads) {
-            const int c = lc[x];
-            const bool real = c != INT_MAX && (int64_t)c < cols;
-            top_score[row * k + x] = real ? ls[x] : __builtin_nanf("");
-            top_index[row * k + x] = real ? c : -1;
-            if (top_matches) top_matches[row * k + x] = real ? matches[row * ld + c] : 0;
         }
         __syncthreads();
     }
+}
+
+// Pass 2: one block per row merges the row's nchunk x k partial hits.
+template <int K>
+__global__ void __launch_bounds__(kTopkThreads)
+k_topk_merge(const float *__restrict__ part_s, const int *__restrict__ part_c,
+             const int32_t *__restrict__ matches, int64_t rows, int64_t cols, int64_t ld, int k,
+             int64_t nchunk, float *__restrict__ top_score, int32_t *__restrict__ top_index,
+             int32_t *__restrict__ top_matches) {
+    __shared__ float ws[kTopkThreads / 64 * K], fs[K];
+    __shared__ int wc[kTopkThreads / 64 * K], fc[K];
+    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        TopList<K> L;
+        L.clear();
+        const int64_t n = nchunk * K;
+        for (int64_t e = threadIdx.x; e < n; e += kTopkThreads) {
+            if ((e % K) < k) L.insert(part_s[row * n + e], part_c[row * n + e]);
+        }
+        block_merge<K>(L, k, ws, wc, fs, fc);
+        write_hits(fs, fc, k, row, cols, ld, matches, top_score, top_index, top_matches);
+        __syncthreads();
+    }
+}
+
+int topk_width(int k) { return k <= 4 ? 4 : (k <= 10 ? 10 : 16); }
+
+int64_t topk_chunks(int64_t cols) {
+    const int64_t n = (cols + kTopkChunk - 1) / kTopkChunk;
+    return n < 1 ? 1 : n;
 }
 
 // Per-row count of score >= threshold (NaN never passes).
@@ -172,27 +279,67 @@ unsigned grid_for(int64_t n, int64_t cap) { return (unsigned)(n < 1 ? 1 : (n > c
 
 using namespace cs;
 
+extern "C" cs_status cs_topk_rows_workspace_size(int64_t rows, int64_t cols, int k,
+                                                 size_t *bytes) {
+    if (rows < 0 || cols < 0) return fail(CS_ERR_INVALID, "bad top-k shape");
+    if (k < 1 || k > kMaxK) return fail(CS_ERR_INVALID, "k must be in [1, %d]", kMaxK);
+    if (!bytes) return fail(CS_ERR_INVALID, "bytes is NULL");
+    const int64_t nchunk = topk_chunks(cols);
+    const int64_t n = nchunk > 1 ? rows * nchunk * topk_width(k) : 0;
+    *bytes = align_up((size_t)n * sizeof(float), 256) + align_up((size_t)n * sizeof(int), 256);
+    return CS_OK;
+}
+
+template <int K>
+void launch_topk(const float *score, const int32_t *matches, int64_t rows, int64_t cols,
+                 int64_t ld, int k, float *top_score, int32_t *top_index, int32_t *top_matches,
+                 float *part_s, int *part_c, int64_t nchunk, hipStream_t hs) {
+    hipLaunchKernelGGL(k_topk_chunks<K>, dim3(grid_for(rows * nchunk, 1 << 20)),
+                       dim3(kTopkThreads), 0, hs, score, matches, rows, cols, ld, k, nchunk,
+                       part_s, part_c, top_score, top_index, top_matches);
+    if (nchunk > 1)
+        hipLaunchKernelGGL(k_topk_merge<K>, dim3(grid_for(rows, 65536)), dim3(kTopkThreads), 0,
+                           hs, part_s, part_c, matches, rows, cols, ld, k, nchunk, top_score,
+                           top_index, top_matches);
+}
+
 extern "C" cs_status cs_topk_rows(const float *score, const int32_t *matches, int64_t rows,
                                   int64_t cols, int64_t ld, int k, float *top_score,
-                                  int32_t *top_index, int32_t *top_matches, void *stream) {
+                                  int32_t *top_index, int32_t *top_matches, void *workspace,
+                                  size_t workspace_bytes, void *stream) {
     if (rows < 0 || cols < 0 || ld < cols) return fail(CS_ERR_INVALID, "bad top-k shape");
     if (k < 1 || k > kMaxK) return fail(CS_ERR_INVALID, "k must be in [1, %d]", kMaxK);
     if (cols > INT_MAX - 1) return fail(CS_ERR_UNSUPPORTED, "more than 2^31 columns");
     if (rows == 0) return CS_OK;
     if (!score || !top_score || !top_index || (top_matches && !matches))
         return fail(CS_ERR_INVALID, "top-k pointer is NULL");
+    size_t need = 0;
+    cs_status st = cs_topk_rows_workspace_size(rows, cols, k, &need);
+    if (st != CS_OK) return st;
+    if (need > 0 && (!workspace || workspace_bytes < need))
+        return fail(CS_ERR_WORKSPACE, "workspace of %zu bytes < %zu required", workspace_bytes,
+                    need);
+    const int64_t nchunk = topk_chunks(cols);
+    const int64_t n = nchunk > 1 ? rows * nchunk * topk_width(k) : 0;
+    Carver c(workspace);
+    float *part_s = c.take<float>(n);
+    int *part_c = c.take<int>(n);
     hipStream_t hs = static_cast<hipStream_t>(stream);
-    const unsigned g = grid_for(rows, 65536);
-    if (k <= 4)
-        hipLaunchKernelGGL(k_topk_rows<4>, dim3(g), dim3(kTopkThreads), 0, hs, score, matches, rows,
-                           cols, ld, k, top_score, top_index, top_matches);
-    else if (k <= 8)
-        hipLaunchKernelGGL(k_topk_rows<8>, dim3(g), dim3(kTopkThreads), 0, hs, score, matches, rows,
-                           cols, ld, k, top_score, top_index, top_matches);
-    else
-        hipLaunchKernelGGL(k_topk_rows<16>, dim3(g), dim3(kTopkThreads), 0, hs, score, matches,
-                           rows, cols, ld, k, top_score, top_index, top_matches);
-    CS_LAUNCH_CHECK("k_topk_rows");
+    switch (topk_width(k)) {
+        case 4:
+            launch_topk<4>(score, matches, rows, cols, ld, k, top_score, top_index, top_matches,
+                           part_s, part_c, nchunk, hs);
+            break;
+        case 10:
+            launch_topk<10>(score, matches, rows, cols, ld, k, top_score, top_index, top_matches,
+                            part_s, part_c, nchunk, hs);
+            break;
+        default:
+            launch_topk<16>(score, matches, rows, cols, ld, k, top_score, top_index, top_matches,
+                            part_s, part_c, nchunk, hs);
+            break;
+    }
+    CS_LAUNCH_CHECK("k_topk");
     return CS_OK;
 }
 

@@ -27,8 +27,11 @@ EXPORTED_SYMBOLS = (
     "cs_version",
     "cs_cosine_greedy_workspace_size",
     "cs_cosine_greedy",
+    "cs_cosine_greedy_indexed_sizes",
+    "cs_cosine_greedy_indexed",
     "cs_fingerprint_workspace_size",
     "cs_fingerprint_similarity",
+    "cs_topk_rows_workspace_size",
     "cs_topk_rows",
     "cs_threshold_compact",
 )
@@ -105,8 +108,17 @@ def load(path: str = LIB_PATH):
     L.cs_fingerprint_similarity.restype = i32
     L.cs_fingerprint_similarity.argtypes = [i32, p, i64, p, i64, i64, p, p, ctypes.c_float, p,
                                             p, i64, p, sz, p]
+    L.cs_cosine_greedy_indexed_sizes.restype = i32
+    L.cs_cosine_greedy_indexed_sizes.argtypes = [ctypes.POINTER(CsSpectra), ctypes.POINTER(sz),
+                                                 ctypes.POINTER(sz)]
+    L.cs_cosine_greedy_indexed.restype = i32
+    L.cs_cosine_greedy_indexed.argtypes = [ctypes.POINTER(CsSpectra), ctypes.POINTER(CsSpectra),
+                                           ctypes.POINTER(CsCosineParams), p, p, p, i64, p, sz, p,
+                                           sz, p]
+    L.cs_topk_rows_workspace_size.restype = i32
+    L.cs_topk_rows_workspace_size.argtypes = [i64, i64, i32, ctypes.POINTER(sz)]
     L.cs_topk_rows.restype = i32
-    L.cs_topk_rows.argtypes = [p, p, i64, i64, i64, i32, p, p, p, p]
+    L.cs_topk_rows.argtypes = [p, p, i64, i64, i64, i32, p, p, p, p, sz, p]
     L.cs_threshold_compact.restype = i32
     L.cs_threshold_compact.argtypes = [p, p, p, i64, i64, i64, ctypes.c_float, i64, p, p, i64,
                                        p, p, p, p, p, p]

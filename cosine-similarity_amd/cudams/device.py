@@ -125,9 +125,11 @@ def fingerprint_similarity(measure: str, ref_words: torch.Tensor, query_words: t
 
 
 def topk_rows(score: torch.Tensor, k: int, matches: Optional[torch.Tensor] = None,
-              rows: Optional[int] = None, cols: Optional[int] = None, stream=None):
+              rows: Optional[int] = None, cols: Optional[int] = None, stream=None,
+              workspace: Optional[Workspace] = None, out=None):
     """Per row, the k best (score desc, column asc): (scores [rows, k] f32,
-    columns [rows, k] i32, matches [rows, k] i32 or None)."""
+    columns [rows, k] i32, matches [rows, k] i32 or None).  ``out`` may hold
+    three preallocated tensors to write into (the third None without matches)."""
     require_gpu()
     L = _native.load()
     rows = score.shape[0] if rows is None else rows
@@ -136,14 +138,70 @@ def topk_rows(score: torch.Tensor, k: int, matches: Optional[torch.Tensor] = Non
         raise ValueError("score must be a row-major float32 tensor")
     if matches is not None and matches.stride(0) != score.stride(0):
         raise ValueError("matches must share the row stride of score")
-    ts = torch.empty((rows, k), dtype=torch.float32, device=score.device)
-    ti = torch.empty((rows, k), dtype=torch.int32, device=score.device)
-    tm = None if matches is None else torch.empty((rows, k), dtype=torch.int32,
-                                                  device=score.device)
+    if out is None:
+        ts = torch.empty((rows, k), dtype=torch.float32, device=score.device)
+        ti = torch.empty((rows, k), dtype=torch.int32, device=score.device)
+        tm = None if matches is None else torch.empty((rows, k), dtype=torch.int32,
+                                                      device=score.device)
+    else:
+        ts, ti, tm = out
+        for t, dt in ((ts, torch.float32), (ti, torch.int32), (tm, torch.int32)):
+            if t is not None and (t.dtype != dt or not t.is_contiguous() or t.numel() < rows * k):
+                raise ValueError("top-k outputs must be contiguous with rows * k elements")
+    n = ctypes.c_size_t(0)
+    _native.check(L.cs_topk_rows_workspace_size(rows, cols, int(k), ctypes.byref(n)))
+    ws = workspace or Workspace(score.device)
+    buf = ws.get(n.value)
     _native.check(L.cs_topk_rows(score.data_ptr(), None if matches is None else matches.data_ptr(),
                                  rows, cols, score.stride(0), int(k), ts.data_ptr(), ti.data_ptr(),
-                                 None if tm is None else tm.data_ptr(), _stream_ptr(stream)))
+                                 None if tm is None else tm.data_ptr(), buf.data_ptr(), buf.numel(),
+                                 _stream_ptr(stream)))
     return ts, ti, tm
+
+
+def cosine_greedy_indexed_bytes(queries: DeviceSpectra):
+    """(index bytes, scratch bytes) of cs_cosine_greedy_indexed for these queries."""
+    L = _native.load()
+    a, b = ctypes.c_size_t(0), ctypes.c_size_t(0)
+    qs = queries.struct()
+    _native.check(L.cs_cosine_greedy_indexed_sizes(ctypes.byref(qs), ctypes.byref(a),
+                                                   ctypes.byref(b)))
+    return int(a.value), int(b.value)
+
+
+def cosine_greedy_indexed(refs: DeviceSpectra, queries: DeviceSpectra, tolerance: float,
+                          shift: float, match_limit: int, score: Optional[torch.Tensor],
+                          matches: Optional[torch.Tensor], overflow: Optional[torch.Tensor],
+                          index: torch.Tensor, scratch: Optional[torch.Tensor], stages: int = 0,
+                          light_candidates: int = 0, round_capacity: int = 0,
+                          stream=None) -> None:
+    """cs_cosine_greedy_indexed: the query index lives in ``index`` (filled by a
+    PREPARE call, then shared read-only), the call's own scratch in
+    ``scratch``.  Outputs may be None for a PREPARE-only call."""
+    require_gpu()
+    L = _native.load()
+    R, Q = refs.n_spectra, queries.n_spectra
+    computes = stages == 0 or (stages & (_native.CS_STAGE_LIGHT | _native.CS_STAGE_HEAVY))
+    ld = 0
+    if computes:
+        for t, dt in ((score, torch.float32), (matches, torch.int32), (overflow, torch.uint8)):
+            if (t is None or t.dtype != dt or not t.is_cuda or t.dim() != 2 or t.shape[0] < R
+                    or t.shape[1] < Q):
+                raise ValueError(f"output must be a CUDA {dt} tensor of shape >= ({R}, {Q})")
+            if t.stride(1) != 1 or t.stride(0) != score.stride(0):
+                raise ValueError("outputs must be row-major with one common row stride")
+        ld = score.stride(0)
+    rs, qs = refs.struct(), queries.struct()
+    p = _native.CsCosineParams(float(tolerance), float(shift), int(match_limit),
+                               int(light_candidates), int(round_capacity), int(stages))
+
+    def ptr(t):
+        return None if t is None else t.data_ptr()
+
+    _native.check(L.cs_cosine_greedy_indexed(
+        ctypes.byref(rs), ctypes.byref(qs), ctypes.byref(p), ptr(score), ptr(matches),
+        ptr(overflow), ld, index.data_ptr(), index.numel(), ptr(scratch),
+        0 if scratch is None else scratch.numel(), _stream_ptr(stream)))
 
 
 def threshold_compact(score: torch.Tensor, matches: torch.Tensor, overflow: torch.Tensor,

@@ -131,6 +131,27 @@ cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *queries,
                            uint8_t *overflow, int64_t ld, void *workspace,
                            size_t workspace_bytes, void *stream);
 
+/*
+ * The same computation with the workspace split in two parts:
+ *   index    the query index, written by CS_STAGE_PREPARE and only read by
+ *            CS_STAGE_LIGHT / CS_STAGE_HEAVY;
+ *   scratch  the per-call heavy-pair queue and counters (LIGHT / HEAVY).
+ * Both sizes depend on `queries` only.  Calls that share one prepared index
+ * and each use their own scratch may run concurrently on different streams:
+ * a device-resident library (the `queries` side, indexed once) searched by
+ * several in-flight batches of request spectra (the `refs` side).  Replaces
+ * the same launcher as cs_cosine_greedy (spectrum_similarity_functions.py:
+ * 264-276) for callers that keep one side resident, which the reference
+ * re-uploads per tile pair (CudaCosineGreedy.py:157-163).
+ */
+cs_status cs_cosine_greedy_indexed_sizes(const cs_spectra *queries, size_t *index_bytes,
+                                         size_t *scratch_bytes);
+
+cs_status cs_cosine_greedy_indexed(const cs_spectra *refs, const cs_spectra *queries,
+                                   const cs_cosine_params *params, float *score, int32_t *matches,
+                                   uint8_t *overflow, int64_t ld, void *index, size_t index_bytes,
+                                   void *scratch, size_t scratch_bytes, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Fingerprint similarity                                                     */
 /* ------------------------------------------------------------------------ */
@@ -175,7 +196,12 @@ cs_status cs_fingerprint_similarity(int measure, const uint32_t *ref_words, int6
  */
 cs_status cs_topk_rows(const float *score, const int32_t *matches, int64_t rows, int64_t cols,
                        int64_t ld, int k, float *top_score, int32_t *top_index,
-                       int32_t *top_matches, void *stream);
+                       int32_t *top_matches, void *workspace, size_t workspace_bytes,
+                       void *stream);
+
+/* Bytes of workspace cs_topk_rows needs (0 when cols <= 4096: rows longer
+ * than that are ranked in 4096-column chunks whose partial hits are merged). */
+cs_status cs_topk_rows_workspace_size(int64_t rows, int64_t cols, int k, size_t *bytes);
 
 /*
  * Threshold compaction (CudaCosineGreedy.py:186-200): entries with

@@ -1,0 +1,80 @@
+"""Top-k per row and threshold compaction on the GPU (libcsamd.so) against
+numpy orderings of the same matrices.  Bar: bit-exact (scores are copied,
+not computed)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def ref_topk(score, k):
+    """(score desc, column asc), NaN last (by column), padded with NaN / -1."""
+    rows, cols = score.shape
+    out_s = np.full((rows, k), np.nan, np.float32)
+    out_i = np.full((rows, k), -1, np.int32)
+    c = np.arange(cols)
+    for r in range(rows):
+        s = score[r]
+        nan = np.isnan(s)
+        order = np.lexsort((c, -np.where(nan, 0, s), nan))
+        take = order[:k]
+        out_s[r, :len(take)] = s[take]
+        out_i[r, :len(take)] = take
+    return out_s, out_i
+
+
+@pytest.mark.parametrize("rows,cols", [(3, 100), (7, 4096), (5, 4097), (4, 20000), (2, 123457)])
+@pytest.mark.parametrize("k", [1, 4, 10, 16])
+def test_topk_rows_matches_numpy_order(gpu, rows, cols, k):
+    import torch
+
+    from cudams.device import topk_rows
+
+    rng = np.random.default_rng(rows * 1000 + cols + k)
+    # coarse values: many exact ties across chunks; some NaN
+    s = (rng.integers(0, 50, size=(rows, cols)) / 49.0).astype(np.float32)
+    s[rng.random((rows, cols)) < 0.01] = np.nan
+    m = rng.integers(0, 100, size=(rows, cols)).astype(np.int32)
+    ld = cols + 3
+    S = torch.full((rows, ld), 5.0, dtype=torch.float32, device="cuda")
+    M = torch.zeros((rows, ld), dtype=torch.int32, device="cuda")
+    S[:, :cols] = torch.from_numpy(s).cuda()
+    M[:, :cols] = torch.from_numpy(m).cuda()
+    ts, ti, tm = topk_rows(S, k, M, rows=rows, cols=cols)
+    torch.cuda.synchronize()
+    ws, wi = ref_topk(s, k)
+    ti = ti.cpu().numpy()
+    np.testing.assert_array_equal(ti, wi)
+    np.testing.assert_array_equal(ts.cpu().numpy(), ws)
+    want_m = np.where(wi >= 0, m[np.arange(rows)[:, None], np.maximum(wi, 0)], 0)
+    np.testing.assert_array_equal(tm.cpu().numpy(), want_m)
+
+
+def test_topk_fewer_columns_than_k(gpu):
+    import torch
+
+    from cudams.device import topk_rows
+
+    s = torch.tensor([[0.5, 0.7, 0.1]], dtype=torch.float32, device="cuda")
+    ts, ti, _ = topk_rows(s, 5)
+    assert ti.cpu().numpy().tolist() == [[1, 0, 2, -1, -1]]
+    assert np.all(np.isnan(ts.cpu().numpy()[0, 3:]))
+
+
+def test_threshold_compact_matches_numpy(gpu):
+    import torch
+
+    from cudams.device import threshold_compact
+
+    rng = np.random.default_rng(4)
+    s = rng.random((37, 1001)).astype(np.float32)
+    m = rng.integers(0, 9, size=s.shape).astype(np.int32)
+    o = (rng.random(s.shape) < 0.1).astype(np.uint8)
+    S, M, O = (torch.from_numpy(x).cuda() for x in (s, m, o))
+    r, c, sc, mm, oo = threshold_compact(S, M, O, 0.9, row0=100)
+    rr, cc = np.nonzero(s >= np.float32(0.9))
+    np.testing.assert_array_equal(r, rr + 100)
+    np.testing.assert_array_equal(c, cc)
+    np.testing.assert_array_equal(sc, s[rr, cc])
+    np.testing.assert_array_equal(mm, m[rr, cc])
+    np.testing.assert_array_equal(oo, o[rr, cc])
