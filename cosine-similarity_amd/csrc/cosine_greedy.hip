@@ -23,6 +23,8 @@
 //                        count is handled exactly in bounded LDS.
 #include <limits.h>
 
+#include <type_traits>
+
 #include "cs_internal.h"
 #include "cg_lane.h"
 #include "cg_tile.h"
@@ -285,6 +287,37 @@ k_tile_dir(int64_t ntiles, const int32_t *tile_np, const int64_t *tile_pbase,
     }
 }
 
+// sint[p] = float32 of the int_pow of list position p, and whether every
+// value of the tile survives that rounding (then the lane kernel forms
+// candidate values from these copies exactly).
+__global__ void __launch_bounds__(256)
+k_tile_values(const double *inp, const int64_t *off, int64_t nq, int64_t ntiles,
+              const int32_t *tile_np, const int64_t *tile_pbase, const uint32_t *swho,
+              float *sint, int32_t *tile_flags) {
+    __shared__ int64_t loff[kTileQueries + 1];
+    const int tid = threadIdx.x;
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int64_t q0 = t * kTileQueries;
+        const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
+        if (tid <= nl) loff[tid] = off[q0 + tid];
+        __syncthreads();
+        const int np = tile_np[t];
+        const int64_t pb = tile_pbase[t];
+        int exact = 1;
+        for (int p = tid; p < np; p += blockDim.x) {
+            const uint32_t w = swho[pb + p];
+            const double x = inp[loff[w >> 16] + (w & 0xffffu)];
+            const float f = (float)x;
+            sint[pb + p] = f;
+            exact &= (double)f == x ? 1 : 0;
+        }
+        if (tid == 0) sint[pb + np] = 0.0f;  // the pad position
+        exact = __syncthreads_and(exact);
+        if (tid == 0) tile_flags[t] = exact;
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Light kernel: one wave per (reference, tile), lanes over reference peaks
 // ---------------------------------------------------------------------------
@@ -297,11 +330,14 @@ struct LightArgs {
     const float *tile_lo, *tile_iw;
     const float *smz;
     const uint32_t *swho;
+    const float *sint;         // query int_pow in sorted-list order, as float32
+    const int32_t *tile_flags; // bit 0: every sint value of the tile is exact
     const int32_t *dir;
-    int lds_cap;         // sorted peaks a tile may hold in LDS
+    int lds_cap;               // sorted peaks a tile may hold in LDS
     float tolc;
     double tol, shift;
-    int exact_from_tile; // query m/z in the sorted list are the exact values
+    int exact_from_tile;       // query m/z in the sorted list are the exact values
+    int no_mz_pow;             // mz_power == 0 on both sides (mz_pow arrays NULL)
     int64_t match_limit;
     float *score;
     int32_t *matches;
@@ -310,7 +346,7 @@ struct LightArgs {
     unsigned long long *heavy_queue;
     int64_t heavy_cap;
     unsigned long long *counters;
-    int64_t n_ref_blocks;  // grid = n_ref_blocks x tiles, reference block fastest
+    int64_t n_ref_blocks;      // grid = n_ref_blocks x tiles, reference block fastest
 };
 
 // Value of lane f (wave-uniform f) without an LDS round trip.
@@ -332,23 +368,29 @@ __device__ __forceinline__ void wave_sync() {
 
 // Per-wave candidate pool of k_light: every exact candidate of the current
 // reference against the tile, linked per query (LIFO; order is irrelevant,
-// the settle step sorts).
-constexpr int kPool = 256;
-constexpr int kMidCap = 64;
-constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first   // candidates a wave settles for one pair in registers
+// the settle step sorts).  A query contributes at most kMidCap entries (more
+// means the pair is settled by the heavy kernel anyway).
+constexpr int kPool = 128;
+constexpr int kMidCap = 64;     // candidates a wave settles for one pair in registers
+constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first
 constexpr size_t kWaveLds =
-    ((size_t)kPool * (8 + 4 + 4 + 1) + (size_t)kTileQueries * 4 * 2 + 16 + kWave * 4 + 15) / 16 * 16;
+    ((size_t)kPool * (8 + 4 + 4 + 1) + (size_t)kTileQueries * (4 * 2 + 1) + 16 + kWave * 4 + 15) /
+    16 * 16;
 constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
 
 // Bytes of dynamic LDS of k_light for a tile capacity `cap`.
 constexpr size_t light_lds_bytes(int cap) {
     return (size_t)kTileQueries * (8 + 8)          // qoff, qnorm
+           + (size_t)(kLightRefs + 1) * 8           // reference offsets
+           + (size_t)kLightRefs * 8                 // reference norms
            + (size_t)kLightWaves * kWaveLds         // pools
            + (size_t)(cap + 1) * 4                  // sorted values (f32)
+           + (size_t)(cap + 1) * 4                  // sorted int_pow (f32)
            + (size_t)kDirPad * 2                    // directory (u16)
            + (size_t)(cap + 1) * 2                  // peak index (u16)
            + (size_t)(cap + 1 + 15) / 16 * 16;      // query lane (u8)
 }
+constexpr int kLightPeakBytes = 4 + 4 + 2 + 1;
 
 // Descending bitonic sort of one key per lane across the wave (64 lanes).
 __device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
@@ -372,13 +414,35 @@ __device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
     }
 }
 
+// s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited on (gfx9 layout:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+// The first two 64-peak chunks of one reference, held in registers from one
+// step before they are probed (lane l holds peaks l and 64 + l).
+struct RefChunk {
+    int rr;        // reference within the workgroup's block (>= nrefs: none)
+    int n;
+    int64_t off;
+    float mz0, mz1;
+    double in0, in1;
+};
+
 // One workgroup = one 64-query tile x kLightRefs references.  The tile's peak
-// index sits in LDS (shared by 16 waves); wave w settles references
-// w, w + 16, ... one at a time: lanes over the reference's peaks probe the
-// index, exact candidates go to the wave's pool, then lane l settles the pair
-// (reference, query l).  The next reference's first 64 peaks are loaded while
-// the current one is settled.
-template <int K>
+// index sits in LDS (shared by 16 waves); wave w settles references taken
+// from a workgroup counter one at a time: lanes over the reference's peaks
+// probe the index, exact candidates go to the wave's pool, then lane l
+// settles the pair (reference, query l).  Each reference's first 128 peaks
+// and int_pow are loaded a whole step before it is probed (two register
+// buffers used in turn).
+//
+// F32: every m/z is its float32 value on both sides, there is no shift and
+// mz_power is 0.  Then the exact window test needs only the float32 values
+// at hand, and a candidate's value ((1 * int_pow_r) * 1) * int_pow_q comes
+// from registers and LDS when the tile's int_pow values are float32 numbers,
+// so a step waits on no global load.  The general variant reads the float64
+// arrays it needs.
+template <int K, bool F32>
 __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     extern __shared__ __align__(16) unsigned char light_lds[];
     __shared__ int next_ref;
@@ -387,7 +451,9 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     // LDS layout (see light_lds_bytes)
     int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
     double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
-    unsigned char *pools = reinterpret_cast<unsigned char *>(qnorm_l + kTileQueries);
+    int64_t *roff_l = reinterpret_cast<int64_t *>(qnorm_l + kTileQueries);
+    double *rnorm_l = reinterpret_cast<double *>(roff_l + kLightRefs + 1);
+    unsigned char *pools = reinterpret_cast<unsigned char *>(rnorm_l + kLightRefs);
     unsigned char *wbase = pools + wave * kWaveLds;
     double *pv = reinterpret_cast<double *>(wbase);
     uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
@@ -397,28 +463,39 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     int *pool_n = head + kTileQueries;
     int *midx = pool_n + 4;
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
+    unsigned char *pdrop = pq + kPool;  // query lost a candidate to a full pool
     float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * kWaveLds);
-    uint16_t *dir_l = reinterpret_cast<uint16_t *>(smz_l + cap + 1);
+    float *sint_l = smz_l + cap + 1;
+    uint16_t *dir_l = reinterpret_cast<uint16_t *>(sint_l + cap + 1);
     uint16_t *wj_l = dir_l + kDirPad;
     unsigned char *wl_l = reinterpret_cast<unsigned char *>(wj_l + cap + 1);
 
-    const int64_t rb = (int64_t)blockIdx.x % a.n_ref_blocks, t = (int64_t)blockIdx.x / a.n_ref_blocks;
+    const int64_t rb = (int64_t)blockIdx.x % a.n_ref_blocks;
+    const int64_t t = (int64_t)blockIdx.x / a.n_ref_blocks;
     const int np = a.tile_np[t];
     const int64_t pb = a.tile_pbase[t];
     const float lo = a.tile_lo[t], iw = a.tile_iw[t];
     const float *gsmz = a.smz + pb;
     const uint32_t *gswho = a.swho + pb;
+    const float *gsint = a.sint + pb;
     const int32_t *gdir = a.dir + t * (kTileBuckets + 1);
     const bool in_lds = np <= cap;
+    // candidate values from float32 copies: exact when mz_power == 0 and the
+    // tile's int_pow values are float32 numbers
+    const bool tile_f32 = F32 && (a.tile_flags[t] & 1);
     const int64_t q0 = t * kTileQueries;
+    const int64_t r0 = rb * kLightRefs;
+    const int nrefs = (int)(r0 + kLightRefs < a.nR ? kLightRefs : a.nR - r0);
+    const SpecView &R = a.R, &Q = a.Q;
     if (in_lds) {  // several independent loads in flight per thread
         for (int e = tid; e <= np; e += 4 * kLightThreads) {
-            float x[4];
+            float x[4], y[4];
             uint32_t w[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const int f = e + u * kLightThreads;
                 x[u] = f <= np ? gsmz[f] : 0.0f;
+                y[u] = F32 && f <= np ? gsint[f] : 0.0f;
                 w[u] = f <= np ? gswho[f] : 0u;
             }
 #pragma unroll
@@ -426,6 +503,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                 const int f = e + u * kLightThreads;
                 if (f <= np) {
                     smz_l[f] = x[u];
+                    if (F32) sint_l[f] = y[u];
                     wj_l[f] = (uint16_t)(w[u] & 0xffffu);
                     wl_l[f] = (unsigned char)(w[u] >> 16);
                 }
@@ -445,57 +523,96 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     }
     if (tid < kTileQueries) {
         const int64_t q = q0 + tid < a.nQ ? q0 + tid : q0;
-        qoff_l[tid] = a.Q.off[q];
-        qnorm_l[tid] = a.Q.norm[q];
+        qoff_l[tid] = Q.off[q];
+        qnorm_l[tid] = Q.norm[q];
     }
+    if (tid <= nrefs) roff_l[tid] = R.off[r0 + tid];
+    if (tid < nrefs) rnorm_l[tid] = R.norm[r0 + tid];
     cnt[lane] = 0;
     head[lane] = -1;
+    pdrop[lane] = 0;
     if (lane == 0) *pool_n = 0;
     if (tid == 0) next_ref = 0;
     __syncthreads();
 
-    const int64_t r0 = rb * kLightRefs;
-    const int nrefs = (int)(r0 + kLightRefs < a.nR ? kLightRefs : a.nR - r0);
-    const SpecView &R = a.R, &Q = a.Q;
     const int64_t q = q0 + lane;
     const bool valid = q < a.nQ;
 
-    // references are taken from a workgroup counter one step ahead, so the
-    // next one's first peaks load while the current one is settled
     auto grab = [&]() {
         int v = 0;
         if (lane == 0) v = atomicAdd(&next_ref, 1);
         return __shfl(v, 0);
     };
-    int rr = grab();
-    int64_t roff_n = 0;
-    int nr_n = 0;
-    float ri_n = 0.0f;
-    if (rr < nrefs) {
-        roff_n = R.off[r0 + rr];
-        nr_n = (int)(R.off[r0 + rr + 1] - roff_n);
-        ri_n = lane < nr_n ? R.mz[roff_n + lane] : 0.0f;
-    }
-    while (rr < nrefs) {
-        const int64_t r = r0 + rr;
-        const int64_t roff = roff_n;
-        const int nr = nr_n;
+    auto fetch = [&](RefChunk &c, int rr) {
+        c.rr = rr;
+        c.n = 0;
+        c.off = 0;
+        if (rr < nrefs) {
+            c.off = roff_l[rr];
+            c.n = (int)(roff_l[rr + 1] - c.off);
+        }
+        const bool h0 = lane < c.n, h1 = lane + kWave < c.n;
+        c.mz0 = h0 ? R.mz[c.off + lane] : 0.0f;
+        c.mz1 = h1 ? R.mz[c.off + kWave + lane] : 0.0f;
+        c.in0 = h0 ? R.inp[c.off + lane] : 0.0;
+        c.in1 = h1 ? R.inp[c.off + kWave + lane] : 0.0;
+    };
 
-        // 1. probe: lanes over the reference's peaks, exact candidates into the pool
-        for (int ib = 0; ib < nr; ib += kWave) {
-            const int i = ib + lane;
-            const float ri = ib == 0 ? ri_n : (i < nr ? R.mz[roff + i] : 0.0f);
-            if (i < nr) {
+    // outputs of the previous reference, held in registers and stored only
+    // after the next probe (a store still in flight would hold up the next
+    // wait for the prefetched peaks)
+    int64_t out_row = -1;
+    float out_score = 0.0f;
+    int out_matches = 0;
+    int out_overflow = 0;
+    auto flush = [&]() {
+        if (out_row >= 0 && valid) {
+            const int64_t o = out_row * a.ld + q;
+            a.score[o] = out_score;
+            a.matches[o] = out_matches;
+            a.overflow[o] = (uint8_t)out_overflow;
+        }
+    };
+
+    auto step = [&](const RefChunk &c) {
+        const int rr = c.rr;
+        const int64_t r = r0 + rr;
+        const int64_t roff = c.off;
+        const int nr = c.n;
+
+        // 1. probe: lanes over the reference's peaks, exact candidates into the
+        // pool.  TF (tile float32 values) is a compile-time branch around the
+        // whole probe, and the two register chunks are probed outside the loop
+        // over further chunks, so the common path has no global load and no
+        // wait in it.
+        auto probe = [&](auto tf) {
+            constexpr bool TF = decltype(tf)::value;
+            auto chunk = [&](int ib, float ri, double rin) {
+                const int i = ib + lane;
+                if (i >= nr) return;
                 const float x0 = ri - a.tolc, x1 = ri + a.tolc;
-                auto emit = [&](int ql, int j, float qv) {
-                    const int64_t qp = qoff_l[ql] + j;
-                    const double mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qp);
-                    const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+                auto emit = [&](int ql, int j, float qv, float qin) {
+                    double mq, mr;
+                    if constexpr (F32) {
+                        mq = (double)qv;
+                        mr = (double)ri;
+                    } else {
+                        mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qoff_l[ql] + j);
+                        mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+                    }
                     if (!exact_window(mr, mq, a.tol, a.shift)) return;
                     if (atomicAdd(&cnt[ql], 1) >= kMidCap) return;  // heavy anyway
                     const int pos = atomicAdd(pool_n, 1);
-                    if (pos >= kPool) return;
-                    pv[pos] = pair_value(R, roff + i, Q, qp);
+                    if (pos >= kPool) {
+                        pdrop[ql] = 1;
+                        return;
+                    }
+                    if constexpr (F32 && TF)  // ((1 * int_pow_r) * 1) * int_pow_q
+                        pv[pos] = rin * (double)qin;
+                    else if constexpr (F32)
+                        pv[pos] = rin * Q.inp[qoff_l[ql] + j];
+                    else
+                        pv[pos] = pair_value(R, roff + i, Q, qoff_l[ql] + j);
                     pij[pos] = pack_ij(i, j);
                     pq[pos] = (unsigned char)ql;
                     pnext[pos] = atomicExch(&head[ql], pos);
@@ -503,32 +620,43 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                 if (in_lds) {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return smz_l[p]; },
                                [&](int b) { return (int)dir_l[b]; },
-                               [&](int p) { emit((int)wl_l[p], (int)wj_l[p], smz_l[p]); });
+                               [&](int p) {
+                                   emit((int)wl_l[p], (int)wj_l[p], smz_l[p],
+                                        TF ? sint_l[p] : 0.0f);
+                               });
                 } else {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return gsmz[p]; },
                                [&](int b) { return (int)gdir[b]; },
                                [&](int p) {
                                    const uint32_t w = gswho[p];
-                                   emit((int)(w >> 16), (int)(w & 0xffffu), gsmz[p]);
+                                   emit((int)(w >> 16), (int)(w & 0xffffu), gsmz[p],
+                                        TF ? gsint[p] : 0.0f);
                                });
                 }
+            };
+            chunk(0, c.mz0, c.in0);
+            if (nr > kWave) chunk(kWave, c.mz1, c.in1);
+            for (int ib = 2 * kWave; ib < nr; ib += kWave) {
+                const int i = ib + lane;
+                const float ri = i < nr ? R.mz[roff + i] : 0.0f;
+                const double rin = i < nr ? R.inp[roff + i] : 0.0;
+                chunk(ib, ri, rin);
             }
-        }
+        };
+        if (tile_f32)
+            probe(std::true_type{});
+        else
+            probe(std::false_type{});
         wave_sync();
+        flush();  // the previous reference's outputs
 
-        // prefetch the next reference while this one is settled
-        const int rn = grab();
-        if (rn < nrefs) {
-            roff_n = R.off[r0 + rn];
-            nr_n = (int)(R.off[r0 + rn + 1] - roff_n);
-            ri_n = lane < nr_n ? R.mz[roff_n + lane] : 0.0f;
-        }
-
-        // 2. settle: lane l = pair (r, query l)
+        // 2. settle: lane l = pair (r, query l); every valid lane ends with its
+        // pair's output in out_*
+        const double rnorm = rnorm_l[rr];
         const int used = *pool_n;
-        const bool over = used > kPool;
         const int n = cnt[lane];
-        const bool complete = !(over && n > 0);  // every candidate of this pair is pooled
+        const bool complete = pdrop[lane] == 0;  // every candidate of this pair is pooled
+        out_row = r;
         // 2a. up to K candidates: walk the pair's list into registers
         const bool small = valid && complete && n <= K;
         int bound = small ? n : 0;  // wave-wide max of the small counts
@@ -536,23 +664,22 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
             const int o2 = __shfl_xor(bound, off);
             bound = o2 > bound ? o2 : bound;
         }
-        LaneCandidates<K> c;
-        c.reset();
+        LaneCandidates<K> cand;
+        cand.reset();
         int p = small ? head[lane] : -1;
         while (__ballot(p >= 0)) {
             if (p >= 0) {
-                c.insert(pv[p], pij[p], bound);
+                cand.insert(pv[p], pij[p], bound);
                 p = pnext[p];
             }
         }
         if (small) {
-            const int64_t o = r * a.ld + q;
             double sc = 0.0;
             int m = 0;
-            c.greedy(sc, m, bound);
-            a.score[o] = n ? (float)(sc / (R.norm[r] * qnorm_l[lane])) : 0.0f;
-            a.matches[o] = m;
-            a.overflow[o] = (uint8_t)(n >= a.match_limit);
+            cand.greedy(sc, m, bound);
+            out_score = n ? (float)(sc / (rnorm * qnorm_l[lane])) : 0.0f;
+            out_matches = m;
+            out_overflow = n >= a.match_limit;
         }
         // 2b. K < n <= 64: the whole wave settles the pair from registers
         unsigned long long mid = __ballot(valid && complete && n > K && n <= kMidCap);
@@ -581,7 +708,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
             wave_sort_desc(v, ij);
             const int vi = (int)(ij >> 16), vj = (int)(ij & 0xffffu);
             bool alive = lane < have;
-            double sc = 0.0;
+            double sc = 0.0;  // the same on every lane (read from lane f)
             int m = 0;
             unsigned long long mk = __ballot(alive);
             while (mk) {
@@ -592,16 +719,14 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                 alive = alive && lane != f && vi != fi && vj != fj;
                 mk = __ballot(alive);
             }
-            if (lane == 0) {
-                const int64_t o = r * a.ld + q0 + sl;
-                a.score[o] = (float)(sc / (R.norm[r] * qnorm_l[sl]));
-                a.matches[o] = m;
-                a.overflow[o] = (uint8_t)(cnt[sl] >= a.match_limit);
+            if (lane == sl) {
+                out_score = (float)(sc / (rnorm * qnorm_l[sl]));
+                out_matches = m;
+                out_overflow = n >= a.match_limit;
             }
         }
         // 2c. the rest go to the heavy kernel
         if (valid && (n > kMidCap || !complete)) {
-            const int64_t o = r * a.ld + q;
             const unsigned long long e = ((unsigned long long)r << 32) | (unsigned int)q;
             if (n > kBigPair) {  // from the top, taken first
                 const unsigned long long slot = atomicAdd(&a.counters[4], 1ull);
@@ -610,17 +735,31 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                 const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
                 if ((int64_t)slot < a.heavy_cap) a.heavy_queue[slot] = e;
             }
-            a.score[o] = 0.0f;
-            a.matches[o] = -1;
-            a.overflow[o] = 0;
+            out_score = 0.0f;
+            out_matches = -1;
+            out_overflow = 0;
         }
         wave_sync();
         cnt[lane] = 0;
         head[lane] = -1;
+        pdrop[lane] = 0;
         if (lane == 0) *pool_n = 0;
         wave_sync();
-        rr = rn;
+    };
+
+    // One wait per step, before the next prefetch is issued: what is then in
+    // flight is this reference's peaks (issued a whole step ago) and the
+    // stores issued after the previous probe.
+    RefChunk nxt;
+    fetch(nxt, grab());
+    for (;;) {
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        const RefChunk cur = nxt;
+        if (cur.rr >= nrefs) break;
+        fetch(nxt, grab());
+        step(cur);
     }
+    flush();
 }
 
 // ---------------------------------------------------------------------------
@@ -1057,6 +1196,8 @@ struct Layout {
     float *tile_lo, *tile_iw, *smz;
     uint32_t *swho;
     int32_t *dir;
+    float *sint;
+    int32_t *tile_flags;
     // per-call scratch (LIGHT / HEAVY)
     unsigned long long *counters, *heavy_queue, *giant_queue;
     int64_t ntiles, heavy_cap;
@@ -1081,6 +1222,8 @@ Layout layout(void *index_base, void *scratch_base, const cs_spectra &Q) {
     L.smz = c.take<float>(Q.n_peaks + T);
     L.swho = c.take<uint32_t>(Q.n_peaks + T);
     L.dir = c.take<int32_t>(T * (kTileBuckets + 1));
+    L.sint = c.take<float>(Q.n_peaks + T);
+    L.tile_flags = c.take<int32_t>(T);
     L.index_bytes = align_up(c.used, 256);
     Carver d(scratch_base);
     L.counters = d.take<unsigned long long>(8);
@@ -1120,24 +1263,34 @@ int light_capacity(int K) {
     }
     const size_t fixed = light_lds_bytes(0) + 64;  // + alignment slack
     const size_t room = (size_t)max_lds > fixed ? (size_t)max_lds - fixed : 0;
-    const int cap = (int)(room / 7) - 1;  // 7 bytes per staged peak
+    const int cap = (int)(room / kLightPeakBytes) - 1;
     return cap < kTileLdsCap ? (cap > 0 ? cap : 0) : kTileLdsCap;
 }
 
-template <int K>
+template <int K, bool F32>
 cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
     a.lds_cap = light_capacity(K);
     const size_t lds = light_lds_bytes(a.lds_cap);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_light<K>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_light<K, F32>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_light<K>, grid, dim3(kLightThreads), lds, st, a);
+    hipLaunchKernelGGL((k_light<K, F32>), grid, dim3(kLightThreads), lds, st, a);
     CS_LAUNCH_CHECK("k_light");
     return CS_OK;
+}
+
+template <bool F32>
+cs_status launch_light_k(int K, dim3 grid, hipStream_t st, const LightArgs &a) {
+    switch (K) {
+        case 1: return launch_light<1, F32>(grid, st, a);
+        case 4: return launch_light<4, F32>(grid, st, a);
+        case 16: return launch_light<16, F32>(grid, st, a);
+        default: return launch_light<8, F32>(grid, st, a);
+    }
 }
 
 // One whole-pipeline or staged call with the index and scratch at two bases.
@@ -1186,6 +1339,10 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
         hipLaunchKernelGGL(k_tile_dir, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, L.ntiles,
                            L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz, L.dir);
         CS_LAUNCH_CHECK("k_tile_dir");
+        hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.inp,
+                           Q.off, nQ, L.ntiles, L.tile_np, L.tile_pbase, L.swho, L.sint,
+                           L.tile_flags);
+        CS_LAUNCH_CHECK("k_tile_values");
     }
     if (nR == 0) return CS_OK;
 
@@ -1210,6 +1367,9 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
         la.smz = L.smz;
         la.swho = L.swho;
         la.dir = L.dir;
+        la.sint = L.sint;
+        la.tile_flags = L.tile_flags;
+        la.no_mz_pow = (Rs.mz_pow == nullptr && Qs.mz_pow == nullptr) ? 1 : 0;
         la.tolc = tolc;
         la.tol = tol;
         la.shift = shift;
@@ -1224,12 +1384,9 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
         la.counters = L.counters;
         la.n_ref_blocks = nrb;
         const dim3 grid((unsigned)(nrb * L.ntiles));
-        switch (K) {
-            case 1: st = launch_light<1>(grid, hs, la); break;
-            case 4: st = launch_light<4>(grid, hs, la); break;
-            case 16: st = launch_light<16>(grid, hs, la); break;
-            default: st = launch_light<8>(grid, hs, la); break;
-        }
+        // float32 path: m/z exact in float32 on both sides, no shift, mz_power 0
+        const bool f32 = la.exact_from_tile && Rs.mz_exact == nullptr && la.no_mz_pow;
+        st = f32 ? launch_light_k<true>(K, grid, hs, la) : launch_light_k<false>(K, grid, hs, la);
         if (st != CS_OK) return st;
     }
     if (!(stages & CS_STAGE_HEAVY)) return CS_OK;
