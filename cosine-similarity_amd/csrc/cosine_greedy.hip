@@ -35,7 +35,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kLightWaves = 16;
 constexpr int kLightThreads = kWave * kLightWaves;
-constexpr int kLightRefs = 256;       // references one light workgroup walks
+constexpr int kMaxLightRefs = 1024;   // largest refs_per_block
 constexpr int kTileLdsCap = 15000;    // sorted tile peaks staged in LDS (larger: read from HBM)
 constexpr int kDefaultK = 8;
 constexpr int kDefaultRound = 256;
@@ -347,7 +347,18 @@ struct LightArgs {
     int64_t heavy_cap;
     unsigned long long *counters;
     int64_t n_ref_blocks;      // grid = n_ref_blocks x tiles, reference block fastest
+    int refs_per_block;        // references one workgroup walks (<= kMaxLightRefs)
 };
+
+// A value every lane of the wave holds equally, moved to a scalar register
+// so that branches and loop bounds on it are scalar.
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    const uint64_t u = (uint64_t)x;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
 
 // Value of lane f (wave-uniform f) without an LDS round trip.
 __device__ __forceinline__ int lane_int(int x, int f) { return __builtin_amdgcn_readlane(x, f); }
@@ -381,8 +392,7 @@ constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
 // Bytes of dynamic LDS of k_light for a tile capacity `cap`.
 constexpr size_t light_lds_bytes(int cap) {
     return (size_t)kTileQueries * (8 + 8)          // qoff, qnorm
-           + (size_t)(kLightRefs + 1) * 8           // reference offsets
-           + (size_t)kLightRefs * 8                 // reference norms
+           + (size_t)(kMaxLightRefs + 1) * 4        // reference offsets (block-relative)
            + (size_t)kLightWaves * kWaveLds         // pools
            + (size_t)(cap + 1) * 4                  // sorted values (f32)
            + (size_t)(cap + 1) * 4                  // sorted int_pow (f32)
@@ -424,11 +434,12 @@ struct RefChunk {
     int rr;        // reference within the workgroup's block (>= nrefs: none)
     int n;
     int64_t off;
+    double norm;
     float mz0, mz1;
     double in0, in1;
 };
 
-// One workgroup = one 64-query tile x kLightRefs references.  The tile's peak
+// One workgroup = one 64-query tile x refs_per_block references.  The tile's peak
 // index sits in LDS (shared by 16 waves); wave w settles references taken
 // from a workgroup counter one at a time: lanes over the reference's peaks
 // probe the index, exact candidates go to the wave's pool, then lane l
@@ -451,9 +462,8 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     // LDS layout (see light_lds_bytes)
     int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
     double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
-    int64_t *roff_l = reinterpret_cast<int64_t *>(qnorm_l + kTileQueries);
-    double *rnorm_l = reinterpret_cast<double *>(roff_l + kLightRefs + 1);
-    unsigned char *pools = reinterpret_cast<unsigned char *>(rnorm_l + kLightRefs);
+    int32_t *roff_l = reinterpret_cast<int32_t *>(qnorm_l + kTileQueries);
+    unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kMaxLightRefs + 1);
     unsigned char *wbase = pools + wave * kWaveLds;
     double *pv = reinterpret_cast<double *>(wbase);
     uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
@@ -484,9 +494,10 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     // tile's int_pow values are float32 numbers
     const bool tile_f32 = F32 && (a.tile_flags[t] & 1);
     const int64_t q0 = t * kTileQueries;
-    const int64_t r0 = rb * kLightRefs;
-    const int nrefs = (int)(r0 + kLightRefs < a.nR ? kLightRefs : a.nR - r0);
+    const int64_t r0 = rb * a.refs_per_block;
+    const int nrefs = (int)(r0 + a.refs_per_block < a.nR ? a.refs_per_block : a.nR - r0);
     const SpecView &R = a.R, &Q = a.Q;
+    const int64_t rbase = R.off[r0];
     if (in_lds) {  // several independent loads in flight per thread
         for (int e = tid; e <= np; e += 4 * kLightThreads) {
             float x[4], y[4];
@@ -526,8 +537,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         qoff_l[tid] = Q.off[q];
         qnorm_l[tid] = Q.norm[q];
     }
-    if (tid <= nrefs) roff_l[tid] = R.off[r0 + tid];
-    if (tid < nrefs) rnorm_l[tid] = R.norm[r0 + tid];
+    for (int x = tid; x <= nrefs; x += kLightThreads) roff_l[x] = (int32_t)(R.off[r0 + x] - rbase);
     cnt[lane] = 0;
     head[lane] = -1;
     pdrop[lane] = 0;
@@ -541,15 +551,18 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     auto grab = [&]() {
         int v = 0;
         if (lane == 0) v = atomicAdd(&next_ref, 1);
-        return __shfl(v, 0);
+        return uniform(v);
     };
     auto fetch = [&](RefChunk &c, int rr) {
         c.rr = rr;
         c.n = 0;
         c.off = 0;
+        c.norm = 1.0;
         if (rr < nrefs) {
-            c.off = roff_l[rr];
-            c.n = (int)(roff_l[rr + 1] - c.off);
+            const int b0 = uniform(roff_l[rr]);
+            c.off = rbase + b0;
+            c.n = uniform(roff_l[rr + 1]) - b0;
+            c.norm = R.norm[r0 + rr];
         }
         const bool h0 = lane < c.n, h1 = lane + kWave < c.n;
         c.mz0 = h0 ? R.mz[c.off + lane] : 0.0f;
@@ -652,8 +665,8 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
 
         // 2. settle: lane l = pair (r, query l); every valid lane ends with its
         // pair's output in out_*
-        const double rnorm = rnorm_l[rr];
-        const int used = *pool_n;
+        const double rnorm = c.norm;
+        const int used = uniform(*pool_n);
         const int n = cnt[lane];
         const bool complete = pdrop[lane] == 0;  // every candidate of this pair is pooled
         out_row = r;
@@ -664,6 +677,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
             const int o2 = __shfl_xor(bound, off);
             bound = o2 > bound ? o2 : bound;
         }
+        bound = uniform(bound);  // scalar: the unrolled steps past it are branched over
         LaneCandidates<K> cand;
         cand.reset();
         int p = small ? head[lane] : -1;
@@ -1293,6 +1307,15 @@ cs_status launch_light_k(int K, dim3 grid, hipStream_t st, const LightArgs &a) {
     }
 }
 
+// References per light workgroup: more amortise the tile staging and the
+// workgroup's tail (its waves wait for the slowest), fewer keep enough
+// workgroups for the chip when there are few query tiles.
+int default_refs_per_block(int64_t nR, int64_t ntiles) {
+    for (int rpb = kMaxLightRefs; rpb > 64; rpb >>= 1)
+        if (ntiles * ((nR + rpb - 1) / rpb) >= 512) return rpb;
+    return 64;
+}
+
 // One whole-pipeline or staged call with the index and scratch at two bases.
 cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
                             const cs_cosine_params *params, float *score, int32_t *matches,
@@ -1346,7 +1369,8 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
     }
     if (nR == 0) return CS_OK;
 
-    const int64_t nrb = (nR + kLightRefs - 1) / kLightRefs;
+    const int rpb = params->refs_per_block ? params->refs_per_block : default_refs_per_block(nR, L.ntiles);
+    const int64_t nrb = (nR + rpb - 1) / rpb;
     if (nrb * L.ntiles > INT_MAX)
         return fail(CS_ERR_UNSUPPORTED, "grid too large (%lld ref blocks x %lld query tiles)",
                     (long long)nrb, (long long)L.ntiles);
@@ -1383,6 +1407,7 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
         la.heavy_cap = L.heavy_cap;
         la.counters = L.counters;
         la.n_ref_blocks = nrb;
+        la.refs_per_block = rpb;
         const dim3 grid((unsigned)(nrb * L.ntiles));
         // float32 path: m/z exact in float32 on both sides, no shift, mz_power 0
         const bool f32 = la.exact_from_tile && Rs.mz_exact == nullptr && la.no_mz_pow;
@@ -1472,6 +1497,8 @@ cs_status check_call(const cs_spectra *refs, const cs_spectra *queries,
     const int T = params->round_capacity ? params->round_capacity : kDefaultRound;
     if (T < 4 || T > 1024 || (T & (T - 1)))
         return fail(CS_ERR_INVALID, "round_capacity must be a power of two in [4, 1024]");
+    if (params->refs_per_block < 0 || params->refs_per_block > kMaxLightRefs)
+        return fail(CS_ERR_INVALID, "refs_per_block must be in [0, %d]", kMaxLightRefs);
     return CS_OK;
 }
 

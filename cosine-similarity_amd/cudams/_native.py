@@ -63,6 +63,7 @@ class CsCosineParams(ctypes.Structure):
         ("match_limit", ctypes.c_int64),
         ("light_candidates", ctypes.c_int32),
         ("round_capacity", ctypes.c_int32),
+        ("refs_per_block", ctypes.c_int32),
         ("stages", ctypes.c_int32),
     ]
 

@@ -58,7 +58,7 @@ def cosine_greedy(refs: DeviceSpectra, queries: DeviceSpectra, tolerance: float,
                   match_limit: int, score: torch.Tensor, matches: torch.Tensor,
                   overflow: torch.Tensor, workspace: Optional[Workspace] = None,
                   light_candidates: int = 0, round_capacity: int = 0, stream=None,
-                  stages: int = 0) -> None:
+                  stages: int = 0, refs_per_block: int = 0) -> None:
     """All pairs into row-major [R, ld] outputs (float32, int32, uint8).
     ``stages`` (a mask of _native.CS_STAGE_*) runs part of the pipeline."""
     require_gpu()
@@ -77,7 +77,8 @@ def cosine_greedy(refs: DeviceSpectra, queries: DeviceSpectra, tolerance: float,
     buf = ws.get(nbytes, keep=reuse)
     rs, qs = refs.struct(), queries.struct()
     p = _native.CsCosineParams(float(tolerance), float(shift), int(match_limit),
-                               int(light_candidates), int(round_capacity), int(stages))
+                               int(light_candidates), int(round_capacity), int(refs_per_block),
+                               int(stages))
     _native.check(L.cs_cosine_greedy(ctypes.byref(rs), ctypes.byref(qs), ctypes.byref(p),
                                      score.data_ptr(), matches.data_ptr(), overflow.data_ptr(),
                                      score.stride(0), buf.data_ptr(), buf.numel(),
@@ -174,7 +175,7 @@ def cosine_greedy_indexed(refs: DeviceSpectra, queries: DeviceSpectra, tolerance
                           matches: Optional[torch.Tensor], overflow: Optional[torch.Tensor],
                           index: torch.Tensor, scratch: Optional[torch.Tensor], stages: int = 0,
                           light_candidates: int = 0, round_capacity: int = 0,
-                          stream=None) -> None:
+                          stream=None, refs_per_block: int = 0) -> None:
     """cs_cosine_greedy_indexed: the query index lives in ``index`` (filled by a
     PREPARE call, then shared read-only), the call's own scratch in
     ``scratch``.  Outputs may be None for a PREPARE-only call."""
@@ -193,7 +194,8 @@ def cosine_greedy_indexed(refs: DeviceSpectra, queries: DeviceSpectra, tolerance
         ld = score.stride(0)
     rs, qs = refs.struct(), queries.struct()
     p = _native.CsCosineParams(float(tolerance), float(shift), int(match_limit),
-                               int(light_candidates), int(round_capacity), int(stages))
+                               int(light_candidates), int(round_capacity), int(refs_per_block),
+                               int(stages))
 
     def ptr(t):
         return None if t is None else t.data_ptr()

@@ -134,7 +134,7 @@ class LibraryIndex:
         import ctypes
 
         qs = self.lib.struct()
-        p = _native.CsCosineParams(self.tolerance, self.shift, self.match_limit, 0, 0,
+        p = _native.CsCosineParams(self.tolerance, self.shift, self.match_limit, 0, 0, 0,
                                    _native.CS_STAGE_LIGHT | _native.CS_STAGE_HEAVY)
         s = (stream or torch.cuda.current_stream(self.device)).cuda_stream
         _native.check(L.cs_cosine_greedy_indexed(

@@ -87,6 +87,8 @@ typedef struct cs_cosine_params {
                                 1, 4, 8, 16; 0 = library default                  */
     int32_t round_capacity;  /* candidates the cooperative path settles per round
                                 (power of two, 4..1024); 0 = library default      */
+    int32_t refs_per_block;  /* references one lane-kernel workgroup walks against
+                                its query tile, 1..1024; 0 = library default      */
     int32_t stages;          /* 0 = whole pipeline, else a mask of CS_STAGE_*:
                                 calls with PREPARE, then LIGHT, then HEAVY on one
                                 stream and one workspace equal one call with 0.
