@@ -34,8 +34,10 @@ namespace cs {
 // Sort key of one candidate: (v, packed (i << 16) | j), compared
 // lexicographically.  Peak indices fit 16 bits (max_peaks <= 65535), so the
 // packed integer orders (i desc, j desc) exactly as the collection order does.
+// Written with non-short-circuit operators so that the device code is
+// branch-free (two compares and a select instead of exec-mask branches).
 CS_HD bool key_greater(double va, uint32_t ia, double vb, uint32_t ib) {
-    return va > vb || (va == vb && ia > ib);
+    return (va > vb) | ((va == vb) & (ia > ib));
 }
 
 CS_HD uint32_t pack_ij(int i, int j) { return (uint32_t(i) << 16) | uint32_t(j); }
@@ -43,7 +45,7 @@ CS_HD uint32_t pack_ij(int i, int j) { return (uint32_t(i) << 16) | uint32_t(j);
 // float64 window test exactly as matchms find_matches writes it.
 CS_HD bool exact_window(double mz_r, double mz_q, double tol, double shift) {
     const double x = mz_q + shift;
-    return !(x > mz_r + tol) && !(x < mz_r - tol);
+    return !(x > mz_r + tol) & !(x < mz_r - tol);
 }
 
 // Up to K candidates kept sorted by key_greater (descending) in registers.
@@ -62,7 +64,7 @@ struct LaneCandidates {
 #pragma unroll
         for (int s = 0; s < K; ++s) {
             if (s >= bound) break;
-            const bool take = (s >= n) || key_greater(cv, cij, v[s], ij[s]);
+            const bool take = (s >= n) | key_greater(cv, cij, v[s], ij[s]);
             const double tv = take ? v[s] : cv;
             const uint32_t tij = take ? ij[s] : cij;
             v[s] = take ? cv : v[s];
@@ -87,8 +89,8 @@ struct LaneCandidates {
             const uint32_t ia = ij[a] >> 16, ja = ij[a] & 0xffffu;
 #pragma unroll
             for (int b = 0; b < a; ++b) {
-                const bool clash = ((ij[b] >> 16) == ia) || ((ij[b] & 0xffffu) == ja);
-                ok = ok && !(acc[b] && clash);
+                const bool clash = ((ij[b] >> 16) == ia) | ((ij[b] & 0xffffu) == ja);
+                ok = ok & !(acc[b] & clash);
             }
             acc[a] = ok;
             if (ok) {

@@ -730,7 +730,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
                 const int fi = lane_int(vi, f), fj = lane_int(vj, f);
                 sc += lane_double(v, f);
                 ++m;
-                alive = alive && lane != f && vi != fi && vj != fj;
+                alive = alive & (lane != f) & (vi != fi) & (vj != fj);
                 mk = __ballot(alive);
             }
             if (lane == sl) {
@@ -846,7 +846,8 @@ __device__ void sort_desc(double *bv, uint32_t *bij, int n) {
 // Greedy over sorted entries [0, n), run by one wave: chunks of 64 in order;
 // inside a chunk the first live candidate is accepted and every later one
 // sharing its i or j is dropped, repeatedly, which is exactly the sequential
-// greedy.  The accepted sum runs in acceptance order.
+// greedy.  A chunk's accepted peaks are marked used after the chunk.  The
+// accepted sum runs in acceptance order.
 __device__ void greedy_sorted(const double *bv, const uint32_t *bij, int n, uint32_t *used_r,
                               uint32_t *used_q, double &s, int &m) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -856,20 +857,21 @@ __device__ void greedy_sorted(const double *bv, const uint32_t *bij, int n, uint
         const uint32_t ij = valid ? bij[idx] : 0u;
         const double v = valid ? bv[idx] : 0.0;
         const int i = (int)(ij >> 16), j = (int)(ij & 0xffffu);
-        bool alive = valid && !bit_set(used_r, i) && !bit_set(used_q, j);
+        bool alive = valid & !bit_set(used_r, i) & !bit_set(used_q, j);
+        bool acc = false;
         unsigned long long mask = __ballot(alive);
         while (mask) {
             const int f = __builtin_ctzll(mask);
             const int fi = lane_int(i, f), fj = lane_int(j, f);
-            const double fv = lane_double(v, f);
-            s += fv;
+            s += lane_double(v, f);
             ++m;
-            if (lane == 0) {
-                used_r[fi >> 5] |= 1u << (fi & 31);
-                used_q[fj >> 5] |= 1u << (fj & 31);
-            }
-            alive = alive && lane != f && i != fi && j != fj;
+            acc = acc | (lane == f);
+            alive = alive & (lane != f) & (i != fi) & (j != fj);
             mask = __ballot(alive);
+        }
+        if (acc) {
+            atomicOr(&used_r[i >> 5], 1u << (i & 31));
+            atomicOr(&used_q[j >> 5], 1u << (j & 31));
         }
         wave_sync();
     }
