@@ -82,27 +82,58 @@ __global__ void k_fill_u64(unsigned long long *p, int64_t n) {
         p[i] = 0ull;
 }
 
+// The query set of one job as the tile kernels see it.  Global tile g of a
+// batch belongs to job tile_job[g] and is that job's tile g - tile_base; its
+// sorted list starts at pos_base + (the job's peaks before the tile) + (its
+// local tile index), one +inf pad per tile.  A single call passes one job and
+// no tile_job array.
+struct PrepJob {
+    const float *mz;
+    const int64_t *off;
+    const double *inp;
+    int64_t nq;
+    float shift32;
+    int64_t tile_base;
+    int64_t pos_base;
+};
+
+__device__ __forceinline__ const PrepJob &prep_job(const PrepJob &one, const PrepJob *jobs,
+                                                  const int32_t *tile_job, int64_t g,
+                                                  int64_t &t) {
+    if (!tile_job) {
+        t = g;
+        return one;
+    }
+    const PrepJob &j = jobs[tile_job[g]];
+    t = g - j.tile_base;
+    return j;
+}
+
 // Per tile: peak count, position of its sorted list, value range.
-__global__ void k_tile_meta(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles,
-                            float shift32, int32_t *tile_np, int64_t *tile_pbase, float *tile_lo,
-                            float *tile_iw) {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
-         t += (int64_t)gridDim.x * blockDim.x) {
+__global__ void k_tile_meta(PrepJob one, const PrepJob *jobs, const int32_t *tile_job,
+                            int64_t ntiles, int32_t *tile_np, int64_t *tile_pbase,
+                            float *tile_lo, float *tile_iw) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ntiles;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        int64_t t;
+        const PrepJob &J = prep_job(one, jobs, tile_job, g, t);
+        const float *mz = J.mz;
+        const int64_t *off = J.off;
         const int64_t q0 = t * kTileQueries;
-        const int64_t q1 = q0 + kTileQueries < nq ? q0 + kTileQueries : nq;
+        const int64_t q1 = q0 + kTileQueries < J.nq ? q0 + kTileQueries : J.nq;
         float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
         for (int64_t q = q0; q < q1; ++q) {
             if (off[q + 1] > off[q]) {
-                const float a = mz[off[q]] + shift32, b = mz[off[q + 1] - 1] + shift32;
+                const float a = mz[off[q]] + J.shift32, b = mz[off[q + 1] - 1] + J.shift32;
                 lo = a < lo ? a : lo;
                 hi = b > hi ? b : hi;
             }
         }
         if (!(lo <= hi)) lo = hi = 0.0f;
-        tile_np[t] = (int32_t)(off[q1] - off[q0]);
-        tile_pbase[t] = off[q0] - off[0] + t;
-        tile_lo[t] = lo;
-        tile_iw[t] = tile_inv_width(lo, hi);
+        tile_np[g] = (int32_t)(off[q1] - off[q0]);
+        tile_pbase[g] = J.pos_base + (off[q0] - off[0]) + t;
+        tile_lo[g] = lo;
+        tile_iw[g] = tile_inv_width(lo, hi);
     }
 }
 
@@ -148,22 +179,25 @@ __device__ void tile_rank_scatter(X xv, const int64_t *loff, int nl, int np, int
 }
 
 __global__ void __launch_bounds__(256)
-k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, float shift32,
+k_tile_sort(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
             const int32_t *tile_np, const int64_t *tile_pbase, float *smz, uint32_t *swho) {
     __shared__ int64_t loff[kTileQueries + 1];
     __shared__ float xs[kSortStage];
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+        int64_t t;
+        const PrepJob &J = prep_job(one, jobs, tile_job, g, t);
+        const float shift32 = J.shift32;
         const int64_t q0 = t * kTileQueries;
-        const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
-        if ((int)threadIdx.x <= nl) loff[threadIdx.x] = off[q0 + threadIdx.x];
+        const int nl = (int)(q0 + kTileQueries < J.nq ? kTileQueries : J.nq - q0);
+        if ((int)threadIdx.x <= nl) loff[threadIdx.x] = J.off[q0 + threadIdx.x];
         __syncthreads();
-        const int np = tile_np[t];
+        const int np = tile_np[g];
         if (np <= kIndexStage) {  // uniform: k_tile_index handles it
             __syncthreads();
             continue;
         }
-        const int64_t pb = tile_pbase[t];
-        const float *src = mz + loff[0];
+        const int64_t pb = tile_pbase[g];
+        const float *src = J.mz + loff[0];
         if (np <= kSortStage) {
             for (int e = threadIdx.x; e < np; e += blockDim.x) xs[e] = src[e] + shift32;
             __syncthreads();
@@ -188,7 +222,7 @@ k_tile_sort(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, flo
 constexpr int kIndexThreads = 1024;
 
 __global__ void __launch_bounds__(kIndexThreads)
-k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, float shift32,
+k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
              const int32_t *tile_np, const int64_t *tile_pbase, const float *tile_lo,
              const float *tile_iw, float *smz, uint32_t *swho, int32_t *dir) {
     extern __shared__ __align__(16) unsigned char index_lds[];
@@ -199,15 +233,19 @@ k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, fl
     float *bx = reinterpret_cast<float *>(cursor + kTileBuckets);
     uint32_t *bw = reinterpret_cast<uint32_t *>(bx + kIndexStage);
     const int tid = threadIdx.x;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int np = tile_np[t];
+    for (int64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+        const int np = tile_np[g];
         if (np > kIndexStage) continue;  // uniform: left to the rank-merge path
+        int64_t t;
+        const PrepJob &J = prep_job(one, jobs, tile_job, g, t);
+        const float *mz = J.mz;
+        const float shift32 = J.shift32;
         const int64_t q0 = t * kTileQueries;
-        const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
-        if (tid <= nl) loff[tid] = off[q0 + tid];
+        const int nl = (int)(q0 + kTileQueries < J.nq ? kTileQueries : J.nq - q0);
+        if (tid <= nl) loff[tid] = J.off[q0 + tid];
         for (int b = tid; b <= kTileBuckets; b += kIndexThreads) start[b] = 0;
         __syncthreads();
-        const float lo = tile_lo[t], iw = tile_iw[t];
+        const float lo = tile_lo[g], iw = tile_iw[g];
         const int64_t g0 = loff[0];
         for (int e = tid; e < np; e += kIndexThreads)
             atomicAdd(&start[tile_bucket(mz[g0 + e] + shift32, lo, iw)], 1);
@@ -238,19 +276,19 @@ k_tile_index(const float *mz, const int64_t *off, int64_t nq, int64_t ntiles, fl
         }
         if (tid == 0) start[kTileBuckets] = np;
         __syncthreads();
-        int32_t *d = dir + t * (kTileBuckets + 1);
+        int32_t *d = dir + g * (kTileBuckets + 1);
         for (int b = tid; b <= kTileBuckets; b += kIndexThreads) d[b] = start[b];
         for (int e = tid; e < np; e += kIndexThreads) {  // scatter into buckets
-            const int64_t g = g0 + e;
+            const int64_t gp = g0 + e;
             int ql = 0;
-            while (ql + 1 < nl && loff[ql + 1] <= g) ++ql;
-            const float x = mz[g] + shift32;
+            while (ql + 1 < nl && loff[ql + 1] <= gp) ++ql;
+            const float x = mz[gp] + shift32;
             const int pos = atomicAdd(&cursor[tile_bucket(x, lo, iw)], 1);
             bx[pos] = x;
-            bw[pos] = tile_who(ql, (int)(g - loff[ql]));
+            bw[pos] = tile_who(ql, (int)(gp - loff[ql]));
         }
         __syncthreads();
-        const int64_t pb = tile_pbase[t];
+        const int64_t pb = tile_pbase[g];
         for (int e = tid; e < np; e += kIndexThreads) {  // rank inside the bucket
             const float x = bx[e];
             const uint32_t w = bw[e];
@@ -291,18 +329,21 @@ k_tile_dir(int64_t ntiles, const int32_t *tile_np, const int64_t *tile_pbase,
 // value of the tile survives that rounding (then the lane kernel forms
 // candidate values from these copies exactly).
 __global__ void __launch_bounds__(256)
-k_tile_values(const double *inp, const int64_t *off, int64_t nq, int64_t ntiles,
+k_tile_values(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
               const int32_t *tile_np, const int64_t *tile_pbase, const uint32_t *swho,
               float *sint, int32_t *tile_flags) {
     __shared__ int64_t loff[kTileQueries + 1];
     const int tid = threadIdx.x;
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+        int64_t t;
+        const PrepJob &J = prep_job(one, jobs, tile_job, g, t);
+        const double *inp = J.inp;
         const int64_t q0 = t * kTileQueries;
-        const int nl = (int)(q0 + kTileQueries < nq ? kTileQueries : nq - q0);
-        if (tid <= nl) loff[tid] = off[q0 + tid];
+        const int nl = (int)(q0 + kTileQueries < J.nq ? kTileQueries : J.nq - q0);
+        if (tid <= nl) loff[tid] = J.off[q0 + tid];
         __syncthreads();
-        const int np = tile_np[t];
-        const int64_t pb = tile_pbase[t];
+        const int np = tile_np[g];
+        const int64_t pb = tile_pbase[g];
         int exact = 1;
         for (int p = tid; p < np; p += blockDim.x) {
             const uint32_t w = swho[pb + p];
@@ -313,7 +354,7 @@ k_tile_values(const double *inp, const int64_t *off, int64_t nq, int64_t ntiles,
         }
         if (tid == 0) sint[pb + np] = 0.0f;  // the pad position
         exact = __syncthreads_and(exact);
-        if (tid == 0) tile_flags[t] = exact;
+        if (tid == 0) tile_flags[g] = exact;
         __syncthreads();
     }
 }
@@ -348,7 +389,31 @@ struct LightArgs {
     unsigned long long *counters;
     int64_t n_ref_blocks;      // grid = n_ref_blocks x tiles, reference block fastest
     int refs_per_block;        // references one workgroup walks (<= kMaxLightRefs)
+    int job;                   // this job's index in a batch
+    int batched;               // heavy-queue entries carry the job index
 };
+
+// A queued pair.  One call: r << 32 | q.  A batch: job << 52 | r << 26 | q
+// (jobs < 4096, references and queries < 2^26 per job).
+__device__ __forceinline__ unsigned long long pair_entry(int batched, int job, int64_t r,
+                                                        int64_t q) {
+    if (batched)
+        return ((unsigned long long)job << 52) | ((unsigned long long)r << 26) |
+               (unsigned long long)q;
+    return ((unsigned long long)r << 32) | (unsigned int)q;
+}
+
+struct PairRef {
+    int job;
+    int64_t r, q;
+};
+
+__device__ __forceinline__ PairRef pair_of(int batched, unsigned long long e) {
+    if (batched)
+        return PairRef{(int)(e >> 52), (int64_t)((e >> 26) & 0x3ffffffull),
+                       (int64_t)(e & 0x3ffffffull)};
+    return PairRef{0, (int64_t)(e >> 32), (int64_t)(e & 0xffffffffull)};
+}
 
 // A value every lane of the wave holds equally, moved to a scalar register
 // so that branches and loop bounds on it are scalar.
@@ -454,7 +519,7 @@ struct RefChunk {
 // so a step waits on no global load.  The general variant reads the float64
 // arrays it needs.
 template <int K, bool F32>
-__global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
+__device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     extern __shared__ __align__(16) unsigned char light_lds[];
     __shared__ int next_ref;
     const int cap = a.lds_cap;
@@ -480,8 +545,6 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     uint16_t *wj_l = dir_l + kDirPad;
     unsigned char *wl_l = reinterpret_cast<unsigned char *>(wj_l + cap + 1);
 
-    const int64_t rb = (int64_t)blockIdx.x % a.n_ref_blocks;
-    const int64_t t = (int64_t)blockIdx.x / a.n_ref_blocks;
     const int np = a.tile_np[t];
     const int64_t pb = a.tile_pbase[t];
     const float lo = a.tile_lo[t], iw = a.tile_iw[t];
@@ -741,7 +804,7 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
         }
         // 2c. the rest go to the heavy kernel
         if (valid && (n > kMidCap || !complete)) {
-            const unsigned long long e = ((unsigned long long)r << 32) | (unsigned int)q;
+            const unsigned long long e = pair_entry(a.batched, a.job, r, q);
             if (n > kBigPair) {  // from the top, taken first
                 const unsigned long long slot = atomicAdd(&a.counters[4], 1ull);
                 if ((int64_t)slot < a.heavy_cap) a.heavy_queue[a.heavy_cap - 1 - slot] = e;
@@ -776,6 +839,33 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     flush();
 }
 
+// One call: workgroup b is (reference block b % n_ref_blocks, tile b / n_ref_blocks).
+template <int K, bool F32>
+__global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
+    light_body<K, F32>(a, (int64_t)blockIdx.x % a.n_ref_blocks,
+                       (int64_t)blockIdx.x / a.n_ref_blocks);
+}
+
+// A batch: a persistent grid takes work items (job << 48 | reference block
+// << 24 | tile), costliest first, from a shared counter until none is left.
+template <int K, bool F32>
+__global__ void __launch_bounds__(kLightThreads)
+k_light_batch(const LightArgs *jobs, const unsigned long long *items, int64_t n_items,
+              unsigned long long *next) {
+    __shared__ unsigned long long item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(next, 1ull);
+        __syncthreads();
+        const unsigned long long it = item_s;
+        __syncthreads();
+        if ((int64_t)it >= n_items) break;
+        const unsigned long long w = items[it];
+        light_body<K, F32>(jobs[w >> 48], (int64_t)((w >> 24) & 0xffffffull),
+                           (int64_t)(w & 0xffffffull));
+        __syncthreads();  // the next item restages the LDS
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Heavy kernel: one workgroup per pair with more than 64 candidates
 // ---------------------------------------------------------------------------
@@ -805,6 +895,8 @@ struct HeavyArgs {
     int64_t heavy_cap;
     unsigned long long *counters;  // [0] heavy count [1] heavy next [2] giant count [3] giant next
     unsigned long long *giant_queue;
+    int job;                       // this job's index in a batch
+    int batched;                   // queue entries carry the job index
 };
 
 __device__ __forceinline__ bool bit_set(const uint32_t *b, int x) {
@@ -982,8 +1074,7 @@ __device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, unsigned ch
     if (C > kHeavyBuf) {
         if (tid == 0) {
             const unsigned long long slot = atomicAdd(&a.counters[2], 1ull);
-            if ((int64_t)slot < a.heavy_cap)
-                a.giant_queue[slot] = ((unsigned long long)r << 32) | (unsigned int)q;
+            if ((int64_t)slot < a.heavy_cap) a.giant_queue[slot] = pair_entry(a.batched, a.job, r, q);
         }
         __syncthreads();
         return;
@@ -1002,7 +1093,10 @@ __device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, unsigned ch
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kHeavyThreads) k_heavy(HeavyArgs a) {
+// Settles the queued pairs: one workgroup per pair, the largest first.  A
+// batch passes its job table (the queue and counters are shared).  When the
+// queue overflowed, every job's matrix is scanned for the -1 sentinel.
+__device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs) {
     extern __shared__ __align__(16) unsigned char heavy_lds[];
     __shared__ int bn_s;
     __shared__ unsigned long long item_s;
@@ -1020,29 +1114,39 @@ __global__ void __launch_bounds__(kHeavyThreads) k_heavy(HeavyArgs a) {
             const unsigned long long item = item_s;
             __syncthreads();
             if (item >= count + big) break;
-            const unsigned long long rq =
+            const unsigned long long e =
                 item < big ? a.heavy_queue[a.heavy_cap - 1 - item] : a.heavy_queue[item - big];
-            heavy_pair(a, (int64_t)(rq >> 32), (int64_t)(rq & 0xffffffffull), heavy_lds, &bn_s);
+            const PairRef p = pair_of(a.batched, e);
+            heavy_pair(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
         }
     } else {  // queue overflowed: find the -1 sentinels
-        const int64_t total = a.nR * a.nQ;
-        for (int64_t c = blockIdx.x; c * kHeavyThreads < total; c += gridDim.x) {
-            if (tid == 0) nlist = 0;
-            __syncthreads();
-            const int64_t p = c * kHeavyThreads + tid;
-            if (p < total) {
-                const int64_t r = p / a.nQ, q = p - r * a.nQ;
-                if (a.matches[r * a.ld + q] == -1) list[atomicAdd(&nlist, 1)] = p;
+        for (int jb = 0; jb < n_jobs; ++jb) {
+            const HeavyArgs &J = jobs ? jobs[jb] : a;
+            const int64_t total = J.nR * J.nQ;
+            for (int64_t c = blockIdx.x; c * kHeavyThreads < total; c += gridDim.x) {
+                if (tid == 0) nlist = 0;
+                __syncthreads();
+                const int64_t p = c * kHeavyThreads + tid;
+                if (p < total) {
+                    const int64_t r = p / J.nQ, q = p - r * J.nQ;
+                    if (J.matches[r * J.ld + q] == -1) list[atomicAdd(&nlist, 1)] = p;
+                }
+                __syncthreads();
+                const int n = nlist;
+                for (int k = 0; k < n; ++k) {
+                    const int64_t pp = list[k];
+                    heavy_pair(J, pp / J.nQ, pp % J.nQ, heavy_lds, &bn_s);
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            const int n = nlist;
-            for (int k = 0; k < n; ++k) {
-                const int64_t pp = list[k];
-                heavy_pair(a, pp / a.nQ, pp % a.nQ, heavy_lds, &bn_s);
-            }
-            __syncthreads();
         }
     }
+}
+
+__global__ void __launch_bounds__(kHeavyThreads) k_heavy(HeavyArgs a) { heavy_body(a, nullptr, 1); }
+
+__global__ void __launch_bounds__(kHeavyThreads) k_heavy_batch(const HeavyArgs *jobs, int n_jobs) {
+    heavy_body(jobs[0], jobs, n_jobs);
 }
 
 // Giant pairs (more than kHeavyBuf candidates): one wave per pair, in rounds.
@@ -1159,7 +1263,7 @@ __device__ void giant_pair(const HeavyArgs &a, int64_t r, int64_t q, uint32_t *u
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kWave) k_giant(HeavyArgs a) {
+__device__ void giant_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs) {
     extern __shared__ __align__(16) unsigned char giant_lds[];
     double *bv = reinterpret_cast<double *>(giant_lds);
     uint32_t *bij = reinterpret_cast<uint32_t *>(bv + a.buf_cap);
@@ -1175,30 +1279,38 @@ __global__ void __launch_bounds__(kWave) k_giant(HeavyArgs a) {
             if (lane == 0) item = atomicAdd(&a.counters[3], 1ull);
             item = __shfl(item, 0);
             if (item >= count) break;
-            const unsigned long long rq = a.giant_queue[item];
-            giant_pair(a, (int64_t)(rq >> 32), (int64_t)(rq & 0xffffffffull), used_r, used_q, bv,
-                       bij);
+            const PairRef p = pair_of(a.batched, a.giant_queue[item]);
+            giant_pair(jobs ? jobs[p.job] : a, p.r, p.q, used_r, used_q, bv, bij);
         }
     } else {  // after k_heavy, the pairs still at -1 are exactly the giant ones
-        const int64_t total = a.nR * a.nQ;
-        for (int64_t c = blockIdx.x; c * kWave < total; c += gridDim.x) {
-            const int64_t p = c * kWave + lane;
-            int64_t r = 0, q = 0;
-            int32_t mv = 0;
-            if (p < total) {
-                r = p / a.nQ;
-                q = p - r * a.nQ;
-                mv = a.matches[r * a.ld + q];
-            }
-            unsigned long long mask = __ballot(p < total && mv == -1);
-            while (mask) {
-                const int f = __builtin_ctzll(mask);
-                const int64_t rf = __shfl(r, f), qf = __shfl(q, f);
-                giant_pair(a, rf, qf, used_r, used_q, bv, bij);
-                mask &= mask - 1ull;
+        for (int jb = 0; jb < n_jobs; ++jb) {
+            const HeavyArgs &J = jobs ? jobs[jb] : a;
+            const int64_t total = J.nR * J.nQ;
+            for (int64_t c = blockIdx.x; c * kWave < total; c += gridDim.x) {
+                const int64_t p = c * kWave + lane;
+                int64_t r = 0, q = 0;
+                int32_t mv = 0;
+                if (p < total) {
+                    r = p / J.nQ;
+                    q = p - r * J.nQ;
+                    mv = J.matches[r * J.ld + q];
+                }
+                unsigned long long mask = __ballot(p < total && mv == -1);
+                while (mask) {
+                    const int f = __builtin_ctzll(mask);
+                    const int64_t rf = __shfl(r, f), qf = __shfl(q, f);
+                    giant_pair(J, rf, qf, used_r, used_q, bv, bij);
+                    mask &= mask - 1ull;
+                }
             }
         }
     }
+}
+
+__global__ void __launch_bounds__(kWave) k_giant(HeavyArgs a) { giant_body(a, nullptr, 1); }
+
+__global__ void __launch_bounds__(kWave) k_giant_batch(const HeavyArgs *jobs, int n_jobs) {
+    giant_body(jobs[0], jobs, n_jobs);
 }
 
 // ---------------------------------------------------------------------------
@@ -1318,6 +1430,41 @@ int default_refs_per_block(int64_t nR, int64_t ntiles) {
     return 64;
 }
 
+// The query-tile index (all of L's tiles), for one job or a batch.
+cs_status launch_prepare(const PrepJob &one, const PrepJob *jobs, const int32_t *tile_job,
+                         const Layout &L, hipStream_t hs) {
+    auto blocks = [](int64_t n, int per) {
+        int64_t b = (n + per - 1) / per;
+        return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+    };
+    hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, one, jobs,
+                       tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw);
+    CS_LAUNCH_CHECK("k_tile_meta");
+    const size_t ilds = sizeof(int) * (2 * kTileBuckets + 1) + 8 * (size_t)kIndexStage;
+    static bool index_attr = false;
+    if (!index_attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_index),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)ilds);
+        if (e != hipSuccess) return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+        index_attr = true;
+    }
+    hipLaunchKernelGGL(k_tile_index, dim3(blocks(L.ntiles, 1)), dim3(kIndexThreads), ilds, hs, one,
+                       jobs, tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw,
+                       L.smz, L.swho, L.dir);
+    CS_LAUNCH_CHECK("k_tile_index");
+    hipLaunchKernelGGL(k_tile_sort, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, one, jobs,
+                       tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.smz, L.swho);
+    CS_LAUNCH_CHECK("k_tile_sort");
+    hipLaunchKernelGGL(k_tile_dir, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, L.ntiles,
+                       L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz, L.dir);
+    CS_LAUNCH_CHECK("k_tile_dir");
+    hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, one, jobs,
+                       tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.swho, L.sint,
+                       L.tile_flags);
+    CS_LAUNCH_CHECK("k_tile_values");
+    return CS_OK;
+}
+
 // One whole-pipeline or staged call with the index and scratch at two bases.
 cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
                             const cs_cosine_params *params, float *score, int32_t *matches,
@@ -1341,33 +1488,9 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
                                       : (CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY);
     if (nQ == 0) return CS_OK;
     if (stages & CS_STAGE_PREPARE) {
-        hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, Q.mz, Q.off,
-                           nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw);
-        CS_LAUNCH_CHECK("k_tile_meta");
-        const size_t ilds = sizeof(int) * (2 * kTileBuckets + 1) + 8 * (size_t)kIndexStage;
-        static bool index_attr = false;
-        if (!index_attr) {
-            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_index),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)ilds);
-            if (e != hipSuccess)
-                return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
-            index_attr = true;
-        }
-        hipLaunchKernelGGL(k_tile_index, dim3(blocks(L.ntiles, 1)), dim3(kIndexThreads), ilds, hs,
-                           Q.mz, Q.off, nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.tile_lo,
-                           L.tile_iw, L.smz, L.swho, L.dir);
-        CS_LAUNCH_CHECK("k_tile_index");
-        hipLaunchKernelGGL(k_tile_sort, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.mz, Q.off,
-                           nQ, L.ntiles, shift32, L.tile_np, L.tile_pbase, L.smz, L.swho);
-        CS_LAUNCH_CHECK("k_tile_sort");
-        hipLaunchKernelGGL(k_tile_dir, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, L.ntiles,
-                           L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz, L.dir);
-        CS_LAUNCH_CHECK("k_tile_dir");
-        hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, Q.inp,
-                           Q.off, nQ, L.ntiles, L.tile_np, L.tile_pbase, L.swho, L.sint,
-                           L.tile_flags);
-        CS_LAUNCH_CHECK("k_tile_values");
+        const PrepJob one{Q.mz, Q.off, Q.inp, nQ, shift32, 0, 0};
+        cs_status pst = launch_prepare(one, nullptr, nullptr, L, hs);
+        if (pst != CS_OK) return pst;
     }
     if (nR == 0) return CS_OK;
 
