@@ -23,7 +23,10 @@
 //                        count is handled exactly in bounded LDS.
 #include <limits.h>
 
+#include <algorithm>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "cs_internal.h"
 #include "cg_lane.h"
@@ -856,10 +859,10 @@ k_light_batch(const LightArgs *jobs, const unsigned long long *items, int64_t n_
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(next, 1ull);
         __syncthreads();
-        const unsigned long long it = item_s;
+        const unsigned long long it = (unsigned long long)uniform64((int64_t)item_s);
         __syncthreads();
         if ((int64_t)it >= n_items) break;
-        const unsigned long long w = items[it];
+        const unsigned long long w = items[it];  // scalar load: `it` is uniform
         light_body<K, F32>(jobs[w >> 48], (int64_t)((w >> 24) & 0xffffffull),
                            (int64_t)(w & 0xffffffull));
         __syncthreads();  // the next item restages the LDS
@@ -1111,7 +1114,7 @@ __device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
         for (;;) {
             if (tid == 0) item_s = atomicAdd(&a.counters[1], 1ull);
             __syncthreads();
-            const unsigned long long item = item_s;
+            const unsigned long long item = (unsigned long long)uniform64((int64_t)item_s);
             __syncthreads();
             if (item >= count + big) break;
             const unsigned long long e =
@@ -1277,7 +1280,7 @@ __device__ void giant_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
         for (;;) {
             unsigned long long item = 0;
             if (lane == 0) item = atomicAdd(&a.counters[3], 1ull);
-            item = __shfl(item, 0);
+            item = (unsigned long long)uniform64((int64_t)__shfl(item, 0));
             if (item >= count) break;
             const PairRef p = pair_of(a.batched, a.giant_queue[item]);
             giant_pair(jobs ? jobs[p.job] : a, p.r, p.q, used_r, used_q, bv, bij);
@@ -1337,9 +1340,18 @@ struct Layout {
 // pairs beyond it are found by scanning for the -1 sentinel instead), so one
 // prepared index serves any reference set, and calls with their own scratch
 // may share it.
+Layout layout_n(void *index_base, void *scratch_base, int64_t ntiles, int64_t npeaks);
+
 Layout layout(void *index_base, void *scratch_base, const cs_spectra &Q) {
+    return layout_n(index_base, scratch_base, (Q.n_spectra + kTileQueries - 1) / kTileQueries,
+                    Q.n_peaks);
+}
+
+// ntiles query tiles holding npeaks peaks in all (one job or a whole batch).
+Layout layout_n(void *index_base, void *scratch_base, int64_t ntiles, int64_t npeaks) {
     Layout L{};
-    L.ntiles = (Q.n_spectra + kTileQueries - 1) / kTileQueries;
+    L.ntiles = ntiles;
+    const cs_spectra Q{0, npeaks, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0.0f};
     L.heavy_cap = kHeavyQueueCap;
     const int64_t T = L.ntiles > 0 ? L.ntiles : 1;
     Carver c(index_base);
@@ -1430,6 +1442,101 @@ int default_refs_per_block(int64_t nR, int64_t ntiles) {
     return 64;
 }
 
+// Lane-kernel arguments of one job; `tiles` is the job's first tile in L.
+LightArgs make_light(const cs_spectra &Rs, const cs_spectra &Qs, double tol, double shift,
+                     int64_t ml, float *score, int32_t *matches, uint8_t *overflow, int64_t ld,
+                     const Layout &L, int64_t tiles, int64_t nrb, int rpb) {
+    LightArgs la{};
+    la.R = view(Rs);
+    la.Q = view(Qs);
+    la.nR = Rs.n_spectra;
+    la.nQ = Qs.n_spectra;
+    la.tile_np = L.tile_np + tiles;
+    la.tile_pbase = L.tile_pbase + tiles;
+    la.tile_lo = L.tile_lo + tiles;
+    la.tile_iw = L.tile_iw + tiles;
+    la.smz = L.smz;
+    la.swho = L.swho;
+    la.dir = L.dir + tiles * (kTileBuckets + 1);
+    la.sint = L.sint;
+    la.tile_flags = L.tile_flags + tiles;
+    la.no_mz_pow = (Rs.mz_pow == nullptr && Qs.mz_pow == nullptr) ? 1 : 0;
+    const float mzmax = Rs.max_abs_mz > Qs.max_abs_mz ? Rs.max_abs_mz : Qs.max_abs_mz;
+    la.tolc = coarse_tolerance(tol, shift, mzmax);
+    la.tol = tol;
+    la.shift = shift;
+    la.exact_from_tile = (Qs.mz_exact == nullptr && shift == 0.0) ? 1 : 0;
+    la.match_limit = ml;
+    la.score = score;
+    la.matches = matches;
+    la.overflow = overflow;
+    la.ld = ld;
+    la.heavy_queue = L.heavy_queue;
+    la.heavy_cap = L.heavy_cap;
+    la.counters = L.counters;
+    la.n_ref_blocks = nrb;
+    la.refs_per_block = rpb;
+    return la;
+}
+
+// float32 path: m/z exact in float32 on both sides, no shift, mz_power 0
+bool light_f32(const LightArgs &la, const cs_spectra &Rs) {
+    return la.exact_from_tile && Rs.mz_exact == nullptr && la.no_mz_pow;
+}
+
+// Cooperative-kernel arguments of one job; r_words / q_words / stage are the
+// LDS shape every job of a launch shares.
+HeavyArgs make_heavy(const cs_spectra &Rs, const cs_spectra &Qs, double tol, double shift,
+                     int64_t ml, float *score, int32_t *matches, uint8_t *overflow, int64_t ld,
+                     const Layout &L, int T, int r_words, int q_words, int stage) {
+    HeavyArgs a{};
+    a.R = view(Rs);
+    a.Q = view(Qs);
+    a.nR = Rs.n_spectra;
+    a.nQ = Qs.n_spectra;
+    const float mzmax = Rs.max_abs_mz > Qs.max_abs_mz ? Rs.max_abs_mz : Qs.max_abs_mz;
+    a.tolc = coarse_tolerance(tol, shift, mzmax);
+    a.shift32 = (float)shift;
+    a.tol = tol;
+    a.shift = shift;
+    a.match_limit = ml;
+    a.round_cap = T;
+    int cap = 1;
+    while (cap < 2 * T + kWave) cap <<= 1;
+    a.buf_cap = cap;
+    a.r_words = r_words;
+    a.q_words = q_words;
+    a.stage = stage;
+    a.exact_from_stage = (Qs.mz_exact == nullptr && shift == 0.0) ? 1 : 0;
+    a.score = score;
+    a.matches = matches;
+    a.overflow = overflow;
+    a.ld = ld;
+    a.heavy_queue = L.heavy_queue;
+    a.heavy_cap = L.heavy_cap;
+    a.counters = L.counters;
+    a.giant_queue = L.giant_queue;
+    return a;
+}
+
+// LDS bytes of k_heavy / k_giant for a shared shape.
+size_t heavy_lds_bytes(int r_words, int q_words, int stage) {
+    const int S = stage ? kHeavyStage : 0;
+    return (size_t)(kHeavyBuf + 2 * S) * 12 + sizeof(uint32_t) * (size_t)(r_words + q_words);
+}
+size_t giant_lds_bytes(int buf_cap, int r_words, int q_words) {
+    return (size_t)buf_cap * (sizeof(double) + sizeof(uint32_t)) +
+           sizeof(uint32_t) * (size_t)(r_words + q_words);
+}
+
+cs_status set_lds_attr(const void *fn, size_t bytes, size_t &have) {
+    if (bytes <= have) return CS_OK;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+    have = bytes;
+    return CS_OK;
+}
+
 // The query-tile index (all of L's tiles), for one job or a batch.
 cs_status launch_prepare(const PrepJob &one, const PrepJob *jobs, const int32_t *tile_job,
                          const Layout &L, hipStream_t hs) {
@@ -1504,86 +1611,29 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
         // the heavy-pair queue of this call starts empty
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)8);
         CS_LAUNCH_CHECK("k_fill_u64");
-        LightArgs la{};
-        la.R = R;
-        la.Q = Q;
-        la.nR = nR;
-        la.nQ = nQ;
-        la.tile_np = L.tile_np;
-        la.tile_pbase = L.tile_pbase;
-        la.tile_lo = L.tile_lo;
-        la.tile_iw = L.tile_iw;
-        la.smz = L.smz;
-        la.swho = L.swho;
-        la.dir = L.dir;
-        la.sint = L.sint;
-        la.tile_flags = L.tile_flags;
-        la.no_mz_pow = (Rs.mz_pow == nullptr && Qs.mz_pow == nullptr) ? 1 : 0;
-        la.tolc = tolc;
-        la.tol = tol;
-        la.shift = shift;
-        la.exact_from_tile = (Qs.mz_exact == nullptr && shift == 0.0) ? 1 : 0;
-        la.match_limit = ml;
-        la.score = score;
-        la.matches = matches;
-        la.overflow = overflow;
-        la.ld = ld;
-        la.heavy_queue = L.heavy_queue;
-        la.heavy_cap = L.heavy_cap;
-        la.counters = L.counters;
-        la.n_ref_blocks = nrb;
-        la.refs_per_block = rpb;
+        const LightArgs la = make_light(Rs, Qs, tol, shift, ml, score, matches, overflow, ld, L,
+                                        0, nrb, rpb);
         const dim3 grid((unsigned)(nrb * L.ntiles));
-        // float32 path: m/z exact in float32 on both sides, no shift, mz_power 0
-        const bool f32 = la.exact_from_tile && Rs.mz_exact == nullptr && la.no_mz_pow;
-        st = f32 ? launch_light_k<true>(K, grid, hs, la) : launch_light_k<false>(K, grid, hs, la);
+        st = light_f32(la, Rs) ? launch_light_k<true>(K, grid, hs, la)
+                               : launch_light_k<false>(K, grid, hs, la);
         if (st != CS_OK) return st;
     }
     if (!(stages & CS_STAGE_HEAVY)) return CS_OK;
 
-    HeavyArgs a{};
-    a.R = R;
-    a.Q = Q;
-    a.nR = nR;
-    a.nQ = nQ;
-    a.tolc = tolc;
-    a.shift32 = shift32;
-    a.tol = tol;
-    a.shift = shift;
-    a.match_limit = ml;
-    a.round_cap = T;
-    int cap = 1;
-    while (cap < 2 * T + kWave) cap <<= 1;
-    a.buf_cap = cap;
-    a.r_words = (Rs.max_peaks + 31) / 32 + 1;
-    a.q_words = (Qs.max_peaks + 31) / 32 + 1;
-    a.stage = (Rs.max_peaks <= kHeavyStage && Qs.max_peaks <= kHeavyStage) ? 1 : 0;
-    a.exact_from_stage = (Qs.mz_exact == nullptr && shift == 0.0) ? 1 : 0;
-    a.score = score;
-    a.matches = matches;
-    a.overflow = overflow;
-    a.ld = ld;
-    a.heavy_queue = L.heavy_queue;
-    a.heavy_cap = L.heavy_cap;
-    a.counters = L.counters;
-    a.giant_queue = L.giant_queue;
-    const int S = a.stage ? kHeavyStage : 0;
-    const size_t hlds = (size_t)(kHeavyBuf + 2 * S) * 12 +
-                        sizeof(uint32_t) * (size_t)(a.r_words + a.q_words);
+    const int r_words = (Rs.max_peaks + 31) / 32 + 1, q_words = (Qs.max_peaks + 31) / 32 + 1;
+    const int stage = (Rs.max_peaks <= kHeavyStage && Qs.max_peaks <= kHeavyStage) ? 1 : 0;
+    const HeavyArgs a = make_heavy(Rs, Qs, tol, shift, ml, score, matches, overflow, ld, L, T,
+                                   r_words, q_words, stage);
+    const size_t hlds = heavy_lds_bytes(r_words, q_words, stage);
     if (hlds > 160 * 1024)
         return fail(CS_ERR_UNSUPPORTED, "max_peaks %d/%d needs %zu bytes of LDS", Rs.max_peaks,
                     Qs.max_peaks, hlds);
     static size_t heavy_attr = 0;
-    if (hlds > heavy_attr) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_heavy),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)hlds);
-        if (e != hipSuccess) return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
-        heavy_attr = hlds;
-    }
+    if ((st = set_lds_attr(reinterpret_cast<const void *>(&k_heavy), hlds, heavy_attr)) != CS_OK)
+        return st;
     hipLaunchKernelGGL(k_heavy, dim3(kHeavyBlocks), dim3(kHeavyThreads), hlds, hs, a);
     CS_LAUNCH_CHECK("k_heavy");
-    const size_t glds = (size_t)cap * (sizeof(double) + sizeof(uint32_t)) +
-                        sizeof(uint32_t) * (size_t)(a.r_words + a.q_words);
+    const size_t glds = giant_lds_bytes(a.buf_cap, r_words, q_words);
     if (glds > 65536)
         return fail(CS_ERR_UNSUPPORTED,
                     "round_capacity %d with max_peaks %d/%d needs %zu bytes of LDS (> 64 KiB)", T,
@@ -1591,6 +1641,49 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
     hipLaunchKernelGGL(k_giant, dim3(kGiantGrid), dim3(kWave), glds, hs, a);
     CS_LAUNCH_CHECK("k_giant");
     return CS_OK;
+}
+
+template <int K, bool F32>
+size_t &light_batch_attr() {
+    static size_t have = 0;
+    return have;
+}
+
+// LDS attribute of the batch lane kernel, set once outside any graph capture.
+template <int K, bool F32>
+cs_status prime_light_batch(int cap) {
+    return set_lds_attr(reinterpret_cast<const void *>(&k_light_batch<K, F32>),
+                        light_lds_bytes(cap), light_batch_attr<K, F32>());
+}
+
+template <bool F32>
+cs_status prime_light_batch_k(int K, int cap) {
+    switch (K) {
+        case 1: return prime_light_batch<1, F32>(cap);
+        case 4: return prime_light_batch<4, F32>(cap);
+        case 16: return prime_light_batch<16, F32>(cap);
+        default: return prime_light_batch<8, F32>(cap);
+    }
+}
+
+template <int K, bool F32>
+void launch_light_batch(unsigned grid, hipStream_t st, const LightArgs *jobs,
+                        const unsigned long long *items, int64_t n, unsigned long long *next,
+                        int cap) {
+    hipLaunchKernelGGL((k_light_batch<K, F32>), dim3(grid), dim3(kLightThreads),
+                       light_lds_bytes(cap), st, jobs, items, n, next);
+}
+
+template <bool F32>
+void launch_light_batch_k(int K, unsigned grid, hipStream_t st, const LightArgs *jobs,
+                          const unsigned long long *items, int64_t n,
+                          unsigned long long *next, int cap) {
+    switch (K) {
+        case 1: launch_light_batch<1, F32>(grid, st, jobs, items, n, next, cap); break;
+        case 4: launch_light_batch<4, F32>(grid, st, jobs, items, n, next, cap); break;
+        case 16: launch_light_batch<16, F32>(grid, st, jobs, items, n, next, cap); break;
+        default: launch_light_batch<8, F32>(grid, st, jobs, items, n, next, cap); break;
+    }
 }
 
 // Argument checks shared by the entry points (no HIP call).
@@ -1693,3 +1786,242 @@ extern "C" cs_status cs_cosine_greedy_indexed(const cs_spectra *refs, const cs_s
     const Layout L = layout(index, scratch, *queries);
     return cosine_greedy_run(refs, queries, params, score, matches, overflow, ld, L, stream);
 }
+
+// ---------------------------------------------------------------------------
+// Batches of independent jobs: one persistent, work-stealing launch per stage
+// ---------------------------------------------------------------------------
+
+struct cs_cosine_batch {
+    std::vector<cs_cosine_job> jobs;
+    int K = kDefaultK, T = kDefaultRound, rpb = 256, f32 = 1, cap = 0;
+    unsigned grid = 0;
+    int r_words = 1, q_words = 1, stage = 1, buf_cap = 0;
+    int64_t ntiles = 0, npeaks = 0;
+    std::vector<int64_t> tile_base, pos_base, nrb;
+    std::vector<int32_t> tile_job;
+    std::vector<unsigned long long> items;
+    size_t off_prep = 0, off_tile_job = 0, off_light = 0, off_heavy = 0, off_items = 0,
+           off_index = 0, off_scratch = 0, bytes = 0, hlds = 0, glds = 0;
+    char *ws = nullptr;  // the uploaded workspace
+    Layout L{};
+};
+
+extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n_jobs,
+                                            const cs_batch_params *params,
+                                            cs_cosine_batch **batch, size_t *workspace_bytes) {
+    if (!batch || !workspace_bytes) return fail(CS_ERR_INVALID, "batch/workspace_bytes is NULL");
+    *batch = nullptr;
+    if (!jobs || n_jobs < 1 || n_jobs > 4095)
+        return fail(CS_ERR_INVALID, "n_jobs must be in [1, 4095] (got %d)", n_jobs);
+    cs_batch_params bp{};
+    if (params) bp = *params;
+    cs_cosine_params cp{};
+    cp.light_candidates = bp.light_candidates;
+    cp.round_capacity = bp.round_capacity;
+    cp.refs_per_block = bp.refs_per_block;
+    if (bp.grid < 0) return fail(CS_ERR_INVALID, "grid must be >= 0");
+    cs_status st;
+    for (int j = 0; j < n_jobs; ++j) {
+        const cs_cosine_job &J = jobs[j];
+        cp.tolerance = J.tolerance;
+        cp.shift = J.shift;
+        cp.match_limit = J.match_limit;
+        if ((st = check_call(&J.refs, &J.queries, &cp, J.score, J.matches, J.overflow, J.ld)) !=
+            CS_OK)
+        {
+            char msg[256];
+            snprintf(msg, sizeof msg, "%s", cs_last_error());
+            return fail(st, "job %d: %s", j, msg);
+        }
+        if (J.refs.n_spectra >= (int64_t(1) << 26) || J.queries.n_spectra >= (int64_t(1) << 26))
+            return fail(CS_ERR_UNSUPPORTED, "job %d: batches take at most 2^26 spectra a side", j);
+    }
+    cs_cosine_batch *b = new cs_cosine_batch();
+    b->jobs.assign(jobs, jobs + n_jobs);
+    b->K = bp.light_candidates ? bp.light_candidates : kDefaultK;
+    b->T = bp.round_capacity ? bp.round_capacity : kDefaultRound;
+    b->rpb = bp.refs_per_block ? bp.refs_per_block : 256;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+        cus = 256;
+    b->grid = bp.grid ? (unsigned)bp.grid : (unsigned)cus;
+    int maxr = 0, maxq = 0;
+    for (int j = 0; j < n_jobs; ++j) {
+        const cs_cosine_job &J = jobs[j];
+        const bool f = J.queries.mz_exact == nullptr && J.shift == 0.0 &&
+                       J.refs.mz_exact == nullptr && J.refs.mz_pow == nullptr &&
+                       J.queries.mz_pow == nullptr;
+        b->f32 = b->f32 && f;
+        maxr = J.refs.max_peaks > maxr ? J.refs.max_peaks : maxr;
+        maxq = J.queries.max_peaks > maxq ? J.queries.max_peaks : maxq;
+        const int64_t nt = (J.queries.n_spectra + kTileQueries - 1) / kTileQueries;
+        b->tile_base.push_back(b->ntiles);
+        b->pos_base.push_back(b->npeaks + b->ntiles);
+        for (int64_t t = 0; t < nt; ++t) b->tile_job.push_back(j);
+        b->ntiles += nt;
+        b->npeaks += J.queries.n_peaks;
+        const int64_t nr = J.refs.n_spectra;
+        b->nrb.push_back(nr && nt ? (nr + b->rpb - 1) / b->rpb : 0);
+    }
+    if (b->ntiles >= (int64_t(1) << 24) * 1) {
+        // tile indices of one job travel in 24 bits; the sum bounds them all
+        bool ok = true;
+        for (int j = 0; j < n_jobs; ++j)
+            ok = ok && (jobs[j].queries.n_spectra + kTileQueries - 1) / kTileQueries < (1 << 24);
+        if (!ok) {
+            delete b;
+            return fail(CS_ERR_UNSUPPORTED, "a job has more than 2^24 query tiles");
+        }
+    }
+    b->r_words = (maxr + 31) / 32 + 1;
+    b->q_words = (maxq + 31) / 32 + 1;
+    b->stage = (maxr <= kHeavyStage && maxq <= kHeavyStage) ? 1 : 0;
+    int cap = 1;
+    while (cap < 2 * b->T + kWave) cap <<= 1;
+    b->buf_cap = cap;
+    b->hlds = heavy_lds_bytes(b->r_words, b->q_words, b->stage);
+    b->glds = giant_lds_bytes(b->buf_cap, b->r_words, b->q_words);
+    if (b->hlds > 160 * 1024 || b->glds > 65536) {
+        delete b;
+        return fail(CS_ERR_UNSUPPORTED, "max_peaks %d/%d with round_capacity %d exceed LDS", maxr,
+                    maxq, cp.round_capacity);
+    }
+    // work items (job, reference block, tile), costliest first: references in
+    // the block x (their mean peaks + one settle per query) x tile share
+    std::vector<std::pair<double, unsigned long long>> w;
+    for (int j = 0; j < n_jobs; ++j) {
+        const cs_cosine_job &J = jobs[j];
+        const int64_t nt = (J.queries.n_spectra + kTileQueries - 1) / kTileQueries;
+        const double mr = J.refs.n_spectra ? (double)J.refs.n_peaks / J.refs.n_spectra : 0.0;
+        for (int64_t rb = 0; rb < b->nrb[j]; ++rb) {
+            const int64_t r0 = rb * b->rpb;
+            const int64_t nref = r0 + b->rpb < J.refs.n_spectra ? b->rpb : J.refs.n_spectra - r0;
+            for (int64_t t = 0; t < nt; ++t) {
+                const int64_t q0 = t * kTileQueries;
+                const int64_t nq = q0 + kTileQueries < J.queries.n_spectra ? kTileQueries
+                                                                          : J.queries.n_spectra - q0;
+                const double cost = (double)nref * (mr + (double)nq) * (1.0 + 10.0 * J.tolerance);
+                w.emplace_back(cost, ((unsigned long long)j << 48) |
+                                         ((unsigned long long)rb << 24) | (unsigned long long)t);
+            }
+        }
+    }
+    std::stable_sort(w.begin(), w.end(),
+                     [](const std::pair<double, unsigned long long> &x,
+                        const std::pair<double, unsigned long long> &y) { return x.first > y.first; });
+    b->items.reserve(w.size());
+    for (auto &x : w) b->items.push_back(x.second);
+    // workspace: tables, then the index and the scratch of all jobs together
+    Carver c(nullptr);
+    c.take<PrepJob>(n_jobs);
+    b->off_tile_job = c.used = align_up(c.used, 256);
+    c.take<int32_t>(b->ntiles + 1);
+    b->off_light = c.used = align_up(c.used, 256);
+    c.take<LightArgs>(n_jobs);
+    b->off_heavy = c.used = align_up(c.used, 256);
+    c.take<HeavyArgs>(n_jobs);
+    b->off_items = c.used = align_up(c.used, 256);
+    c.take<unsigned long long>(b->items.size() + 1);
+    b->off_index = align_up(c.used, 256);
+    const Layout need = layout_n(nullptr, nullptr, b->ntiles, b->npeaks);
+    b->off_scratch = b->off_index + need.index_bytes;
+    b->bytes = b->off_scratch + need.scratch_bytes;
+    // LDS attributes now, so that a run can be captured into a graph
+    b->cap = light_capacity(b->K);
+    static size_t heavy_attr_b = 0, giant_attr_b = 0;
+    if ((st = b->f32 ? prime_light_batch_k<true>(b->K, b->cap)
+                     : prime_light_batch_k<false>(b->K, b->cap)) != CS_OK ||
+        (st = set_lds_attr(reinterpret_cast<const void *>(&k_heavy_batch), b->hlds,
+                           heavy_attr_b)) != CS_OK ||
+        (st = set_lds_attr(reinterpret_cast<const void *>(&k_giant_batch), b->glds,
+                           giant_attr_b)) != CS_OK) {
+        delete b;
+        return st;
+    }
+    *batch = b;
+    *workspace_bytes = b->bytes;
+    return CS_OK;
+}
+
+extern "C" cs_status cs_cosine_batch_upload(cs_cosine_batch *b, void *workspace,
+                                            size_t workspace_bytes, void *stream) {
+    if (!b) return fail(CS_ERR_INVALID, "batch is NULL");
+    if (!workspace || workspace_bytes < b->bytes)
+        return fail(CS_ERR_WORKSPACE, "workspace of %zu bytes < %zu required", workspace_bytes,
+                    b->bytes);
+    char *ws = static_cast<char *>(workspace);
+    b->L = layout_n(ws + b->off_index, ws + b->off_scratch, b->ntiles, b->npeaks);
+    const int n = (int)b->jobs.size();
+    std::vector<char> host(b->off_index, 0);
+    PrepJob *prep = reinterpret_cast<PrepJob *>(host.data());
+    int32_t *tile_job = reinterpret_cast<int32_t *>(host.data() + b->off_tile_job);
+    LightArgs *light = reinterpret_cast<LightArgs *>(host.data() + b->off_light);
+    HeavyArgs *heavy = reinterpret_cast<HeavyArgs *>(host.data() + b->off_heavy);
+    unsigned long long *items = reinterpret_cast<unsigned long long *>(host.data() + b->off_items);
+    for (int j = 0; j < n; ++j) {
+        const cs_cosine_job &J = b->jobs[j];
+        const SpecView Q = view(J.queries);
+        prep[j] = PrepJob{Q.mz, Q.off, Q.inp, J.queries.n_spectra, (float)J.shift,
+                          b->tile_base[j], b->pos_base[j]};
+        LightArgs la = make_light(J.refs, J.queries, J.tolerance, J.shift, J.match_limit, J.score,
+                                  J.matches, J.overflow, J.ld, b->L, b->tile_base[j], b->nrb[j],
+                                  b->rpb);
+        la.lds_cap = b->cap;
+        la.job = j;
+        la.batched = 1;
+        light[j] = la;
+        HeavyArgs ha = make_heavy(J.refs, J.queries, J.tolerance, J.shift, J.match_limit, J.score,
+                                  J.matches, J.overflow, J.ld, b->L, b->T, b->r_words, b->q_words,
+                                  b->stage);
+        ha.job = j;
+        ha.batched = 1;
+        heavy[j] = ha;
+    }
+    for (size_t t = 0; t < b->tile_job.size(); ++t) tile_job[t] = b->tile_job[t];
+    for (size_t k = 0; k < b->items.size(); ++k) items[k] = b->items[k];
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemcpyAsync(ws, host.data(), host.size(), hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipStreamSynchronize(hs);
+    if (e != hipSuccess) return fail(CS_ERR_HIP, "batch upload: %s", hipGetErrorString(e));
+    b->ws = ws;
+    return CS_OK;
+}
+
+extern "C" cs_status cs_cosine_batch_run(const cs_cosine_batch *b, void *stream) {
+    if (!b) return fail(CS_ERR_INVALID, "batch is NULL");
+    if (!b->ws) return fail(CS_ERR_INVALID, "batch is not uploaded");
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    const Layout &L = b->L;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)8);
+    CS_LAUNCH_CHECK("k_fill_u64");
+    const PrepJob *prep = reinterpret_cast<const PrepJob *>(b->ws);
+    const int32_t *tile_job = reinterpret_cast<const int32_t *>(b->ws + b->off_tile_job);
+    const LightArgs *light = reinterpret_cast<const LightArgs *>(b->ws + b->off_light);
+    const HeavyArgs *heavy = reinterpret_cast<const HeavyArgs *>(b->ws + b->off_heavy);
+    const unsigned long long *items =
+        reinterpret_cast<const unsigned long long *>(b->ws + b->off_items);
+    if (b->ntiles > 0) {
+        cs_status st = launch_prepare(PrepJob{}, prep, tile_job, L, hs);
+        if (st != CS_OK) return st;
+    }
+    const int64_t n_items = (int64_t)b->items.size();
+    if (n_items == 0) return CS_OK;
+    const unsigned grid = (unsigned)std::min<int64_t>(b->grid, n_items);
+    if (b->f32)
+        launch_light_batch_k<true>(b->K, grid, hs, light, items, n_items, &L.counters[5], b->cap);
+    else
+        launch_light_batch_k<false>(b->K, grid, hs, light, items, n_items, &L.counters[5], b->cap);
+    CS_LAUNCH_CHECK("k_light_batch");
+    const int n = (int)b->jobs.size();
+    hipLaunchKernelGGL(k_heavy_batch, dim3(kHeavyBlocks), dim3(kHeavyThreads), b->hlds, hs, heavy,
+                       n);
+    CS_LAUNCH_CHECK("k_heavy_batch");
+    hipLaunchKernelGGL(k_giant_batch, dim3(kGiantGrid), dim3(kWave), b->glds, hs, heavy, n);
+    CS_LAUNCH_CHECK("k_giant_batch");
+    return CS_OK;
+}
+
+extern "C" void cs_cosine_batch_destroy(cs_cosine_batch *b) { delete b; }
+

@@ -29,6 +29,10 @@ EXPORTED_SYMBOLS = (
     "cs_cosine_greedy",
     "cs_cosine_greedy_indexed_sizes",
     "cs_cosine_greedy_indexed",
+    "cs_cosine_batch_create",
+    "cs_cosine_batch_upload",
+    "cs_cosine_batch_run",
+    "cs_cosine_batch_destroy",
     "cs_fingerprint_workspace_size",
     "cs_fingerprint_similarity",
     "cs_topk_rows_workspace_size",
@@ -69,6 +73,33 @@ class CsCosineParams(ctypes.Structure):
 
 
 CS_STAGE_PREPARE, CS_STAGE_LIGHT, CS_STAGE_HEAVY = 1, 2, 4
+
+
+class CsCosineJob(ctypes.Structure):
+    """struct cs_cosine_job."""
+
+    _fields_ = [
+        ("refs", CsSpectra),
+        ("queries", CsSpectra),
+        ("tolerance", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("match_limit", ctypes.c_int64),
+        ("score", ctypes.c_void_p),
+        ("matches", ctypes.c_void_p),
+        ("overflow", ctypes.c_void_p),
+        ("ld", ctypes.c_int64),
+    ]
+
+
+class CsBatchParams(ctypes.Structure):
+    """struct cs_batch_params."""
+
+    _fields_ = [
+        ("light_candidates", ctypes.c_int32),
+        ("round_capacity", ctypes.c_int32),
+        ("refs_per_block", ctypes.c_int32),
+        ("grid", ctypes.c_int32),
+    ]
 
 
 class NativeError(RuntimeError):
@@ -120,6 +151,16 @@ def load(path: str = LIB_PATH):
     L.cs_topk_rows_workspace_size.argtypes = [i64, i64, i32, ctypes.POINTER(sz)]
     L.cs_topk_rows.restype = i32
     L.cs_topk_rows.argtypes = [p, p, i64, i64, i64, i32, p, p, p, p, sz, p]
+    L.cs_cosine_batch_create.restype = i32
+    L.cs_cosine_batch_create.argtypes = [ctypes.POINTER(CsCosineJob), ctypes.c_int32,
+                                         ctypes.POINTER(CsBatchParams),
+                                         ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(sz)]
+    L.cs_cosine_batch_upload.restype = i32
+    L.cs_cosine_batch_upload.argtypes = [p, p, sz, p]
+    L.cs_cosine_batch_run.restype = i32
+    L.cs_cosine_batch_run.argtypes = [p, p]
+    L.cs_cosine_batch_destroy.restype = None
+    L.cs_cosine_batch_destroy.argtypes = [p]
     L.cs_threshold_compact.restype = i32
     L.cs_threshold_compact.argtypes = [p, p, p, i64, i64, i64, ctypes.c_float, i64, p, p, i64,
                                        p, p, p, p, p, p]

@@ -155,6 +155,64 @@ cs_status cs_cosine_greedy_indexed(const cs_spectra *refs, const cs_spectra *que
                                    void *scratch, size_t scratch_bytes, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batches of independent CosineGreedy matrices                               */
+/* ------------------------------------------------------------------------ */
+
+/* One matrix of a batch: the arguments of one cs_cosine_greedy call. */
+typedef struct cs_cosine_job {
+    cs_spectra refs;
+    cs_spectra queries;
+    double tolerance;
+    double shift;
+    int64_t match_limit;
+    float *score;            /* device [refs.n_spectra x ld] */
+    int32_t *matches;
+    uint8_t *overflow;
+    int64_t ld;
+} cs_cosine_job;
+
+/* Launch shape of a batch (what an autotuner chooses); 0 = library default. */
+typedef struct cs_batch_params {
+    int32_t light_candidates;  /* as cs_cosine_params.light_candidates          */
+    int32_t round_capacity;    /* as cs_cosine_params.round_capacity            */
+    int32_t refs_per_block;    /* references of one work item (x 64 queries), 1..1024;
+                                  default 256                                     */
+    int32_t grid;              /* persistent workgroups of the lane kernel;
+                                  default one per compute unit                    */
+} cs_batch_params;
+
+typedef struct cs_cosine_batch cs_cosine_batch;  /* host-side plan, opaque */
+
+/*
+ * Plans a batch of n_jobs (1..4095) independent matrices of uneven sizes and
+ * tolerances: validates every job as cs_cosine_greedy would, splits all of
+ * them into work items (job, 64-query tile, block of references), orders the
+ * items costliest first and sizes one workspace for everything.  Replaces the
+ * reference's one launch per tile pair and host sync per tile
+ * (CudaCosineGreedy.py:149-185), repeated per matrix; the reference has no
+ * batch entry point.  The plan is host memory: free it with
+ * cs_cosine_batch_destroy.
+ */
+cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n_jobs,
+                                 const cs_batch_params *params, cs_cosine_batch **batch,
+                                 size_t *workspace_bytes);
+
+/* Copies the plan's tables into `workspace` (synchronises `stream`). */
+cs_status cs_cosine_batch_upload(cs_cosine_batch *batch, void *workspace, size_t workspace_bytes,
+                                 void *stream);
+
+/*
+ * Computes every job: query tiling of all jobs, one persistent lane-kernel
+ * launch whose workgroups take work items from a shared counter, then one
+ * launch each for the queued many-candidate pairs.  Only kernel launches on
+ * `stream` (no copies, no memsets, no host sync), so it can be captured into
+ * a HIP graph and replayed; results equal cs_cosine_greedy's per job.
+ */
+cs_status cs_cosine_batch_run(const cs_cosine_batch *batch, void *stream);
+
+void cs_cosine_batch_destroy(cs_cosine_batch *batch);
+
+/* ------------------------------------------------------------------------ */
 /* Fingerprint similarity                                                     */
 /* ------------------------------------------------------------------------ */
 
