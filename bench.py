@@ -317,6 +317,56 @@ def bench_serving(a) -> dict:
     return out
 
 
+def bench_batched(a) -> dict:
+    """64 independent CosineGreedy matrices (seed 6): references and queries
+    log-uniform in [16, 2048] x [16, 4096], lognormal peaks, tolerance 0.01 or
+    0.1 per job, mz_power 0.5.  The autotuner picks the launch shape on the
+    batch itself; then the graph-captured batch (one persistent lane-kernel
+    launch for all jobs) is timed against one cs_cosine_greedy call per job,
+    both with HIP events on the same stream."""
+    import numpy as np
+    import torch
+
+    from cudams import synthetic
+    from cudams.batch import BatchRunner, _DeviceJob, autotune, per_job, time_runner
+    from cudams.device import Workspace
+
+    jobs = synthetic.batched_jobs(seed=6)
+    pairs = sum(j.refs.n_spectra * j.queries.n_spectra for j in jobs)
+    log(f"batched: {len(jobs)} jobs, {pairs} pairs; autotuning")
+    t0 = time.perf_counter()
+    best, timings = autotune(jobs, replays=3)
+    tune_s = time.perf_counter() - t0
+    runner = BatchRunner(jobs, best)
+    graph_ms = time_runner(runner, replays=max(3, a.steps))
+    dev = runner.device
+    ref = [_DeviceJob(j, dev) for j in jobs]
+    ws = Workspace(dev)
+    per_job(ref, ws)  # warm
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    reps = max(3, a.steps)
+    e0.record()
+    for _ in range(reps):
+        per_job(ref, ws)
+    e1.record()
+    e1.synchronize()
+    per_ms = e0.elapsed_time(e1) / reps
+    runner.run()
+    same = all(all(np.array_equal(x, y) for x, y in zip(g, w))
+               for g, w in zip(runner.results(), [d.result() for d in ref]))
+    runner.close()
+    log(f"batched: graph {graph_ms:.3f} ms, per-job launches {per_ms:.3f} ms, tuned {best}")
+    return {"jobs": len(jobs), "pairs": pairs, "mz_power": 0.5, "tolerances": [0.01, 0.1],
+            "graph_ms": graph_ms, "graph_pairs_per_s": pairs / (graph_ms / 1e3),
+            "per_job_ms": per_ms, "per_job_pairs_per_s": pairs / (per_ms / 1e3),
+            "speedup": per_ms / graph_ms, "identical_to_per_job": bool(same),
+            "tuned": best.__dict__, "tuning_s": tune_s,
+            "tuning_ms": {f"K{p.light_candidates}/rpb{p.refs_per_block}": round(v, 4)
+                          for p, v in timings.items()}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +378,8 @@ def main() -> int:
     ap.add_argument("--round-capacity", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-jobs section of the cosine line")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the served-search section of the cosine line")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
@@ -344,6 +396,8 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     res = {"cosine": bench_cosine, "fingerprint": bench_fingerprint, "copy": bench_copy}[a.workload](a)
+    if a.workload == "cosine" and not a.no_batched:
+        res["batched_jobs"] = bench_batched(a)
     if a.workload == "cosine" and not a.no_serving:
         res["serving"] = bench_serving(a)
     print(json.dumps(res), flush=True)

@@ -659,18 +659,20 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
         const int64_t roff = c.off;
         const int nr = c.n;
 
-        // 1. probe: lanes over the reference's peaks, exact candidates into the
-        // pool.  TF (tile float32 values) is a compile-time branch around the
-        // whole probe, and the two register chunks are probed outside the loop
-        // over further chunks, so the common path has no global load and no
-        // wait in it.
-        auto probe = [&](auto tf) {
+        // 1. probe: lanes over the reference's peaks, exact candidates of the
+        // queries in `filter` into the pool.  TF (tile float32 values) is a
+        // compile-time branch around the whole probe, and the two register
+        // chunks are probed outside the loop over further chunks, so the
+        // common path has no global load and no wait in it.
+        auto probe = [&](auto tf, auto from_regs, unsigned long long filter) {
             constexpr bool TF = decltype(tf)::value;
+            constexpr bool REGS = decltype(from_regs)::value;
             auto chunk = [&](int ib, float ri, double rin) {
                 const int i = ib + lane;
                 if (i >= nr) return;
                 const float x0 = ri - a.tolc, x1 = ri + a.tolc;
                 auto emit = [&](int ql, int j, float qv, float qin) {
+                    if (!((filter >> ql) & 1ull)) return;
                     double mq, mr;
                     if constexpr (F32) {
                         mq = (double)qv;
@@ -713,100 +715,20 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                                });
                 }
             };
-            chunk(0, c.mz0, c.in0);
-            if (nr > kWave) chunk(kWave, c.mz1, c.in1);
-            for (int ib = 2 * kWave; ib < nr; ib += kWave) {
+            int from = 0;
+            if constexpr (REGS) {
+                chunk(0, c.mz0, c.in0);
+                if (nr > kWave) chunk(kWave, c.mz1, c.in1);
+                from = 2 * kWave;
+            }
+            for (int ib = from; ib < nr; ib += kWave) {
                 const int i = ib + lane;
                 const float ri = i < nr ? R.mz[roff + i] : 0.0f;
                 const double rin = i < nr ? R.inp[roff + i] : 0.0;
                 chunk(ib, ri, rin);
             }
         };
-        if (tile_f32)
-            probe(std::true_type{});
-        else
-            probe(std::false_type{});
-        wave_sync();
-        flush();  // the previous reference's outputs
-
-        // 2. settle: lane l = pair (r, query l); every valid lane ends with its
-        // pair's output in out_*
-        const double rnorm = c.norm;
-        const int used = uniform(*pool_n);
-        const int n = cnt[lane];
-        const bool complete = pdrop[lane] == 0;  // every candidate of this pair is pooled
-        out_row = r;
-        // 2a. up to K candidates: walk the pair's list into registers
-        const bool small = valid && complete && n <= K;
-        int bound = small ? n : 0;  // wave-wide max of the small counts
-        for (int off = kWave / 2; off > 0; off >>= 1) {
-            const int o2 = __shfl_xor(bound, off);
-            bound = o2 > bound ? o2 : bound;
-        }
-        bound = uniform(bound);  // scalar: the unrolled steps past it are branched over
-        LaneCandidates<K> cand;
-        cand.reset();
-        int p = small ? head[lane] : -1;
-        while (__ballot(p >= 0)) {
-            if (p >= 0) {
-                cand.insert(pv[p], pij[p], bound);
-                p = pnext[p];
-            }
-        }
-        if (small) {
-            double sc = 0.0;
-            int m = 0;
-            cand.greedy(sc, m, bound);
-            out_score = n ? (float)(sc / (rnorm * qnorm_l[lane])) : 0.0f;
-            out_matches = m;
-            out_overflow = n >= a.match_limit;
-        }
-        // 2b. K < n <= 64: the whole wave settles the pair from registers
-        unsigned long long mid = __ballot(valid && complete && n > K && n <= kMidCap);
-        while (mid) {
-            const int sl = __builtin_ctzll(mid);
-            mid &= mid - 1ull;
-            int have = 0;
-            const int lim = used < kPool ? used : kPool;
-            for (int base = 0; base < lim; base += kWave) {  // compact the pair's entries
-                const int e = base + lane;
-                const bool mine = e < lim && pq[e] == sl;
-                const unsigned long long mk = __ballot(mine);
-                // at most kMidCap entries belong to sl, so every slot is < 64
-                if (mine) midx[have + __popcll(mk & ((1ull << lane) - 1ull))] = e;
-                have += __popcll(mk);
-            }
-            wave_sync();
-            double v = -__builtin_huge_val();
-            uint32_t ij = 0u;
-            if (lane < have) {
-                const int e = midx[lane];
-                v = pv[e];
-                ij = pij[e];
-            }
-            wave_sync();
-            wave_sort_desc(v, ij);
-            const int vi = (int)(ij >> 16), vj = (int)(ij & 0xffffu);
-            bool alive = lane < have;
-            double sc = 0.0;  // the same on every lane (read from lane f)
-            int m = 0;
-            unsigned long long mk = __ballot(alive);
-            while (mk) {
-                const int f = __builtin_ctzll(mk);
-                const int fi = lane_int(vi, f), fj = lane_int(vj, f);
-                sc += lane_double(v, f);
-                ++m;
-                alive = alive & (lane != f) & (vi != fi) & (vj != fj);
-                mk = __ballot(alive);
-            }
-            if (lane == sl) {
-                out_score = (float)(sc / (rnorm * qnorm_l[sl]));
-                out_matches = m;
-                out_overflow = n >= a.match_limit;
-            }
-        }
-        // 2c. the rest go to the heavy kernel
-        if (valid && (n > kMidCap || !complete)) {
+        auto queue_heavy = [&](int n) {
             const unsigned long long e = pair_entry(a.batched, a.job, r, q);
             if (n > kBigPair) {  // from the top, taken first
                 const unsigned long long slot = atomicAdd(&a.counters[4], 1ull);
@@ -818,13 +740,139 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             out_score = 0.0f;
             out_matches = -1;
             out_overflow = 0;
+        };
+
+        // Pass 0 probes for every query.  A query whose candidates did not all
+        // fit the pool (pdrop) but are at most kMidCap is probed again in a
+        // later pass with a group of such queries: groups are cut from the
+        // exclusive prefix sum of their counts in steps of kMidCap, so one
+        // group's candidates fill at most 2 * kMidCap = kPool entries.
+        static_assert(2 * kMidCap <= kPool, "a group of pending queries fits the pool");
+        const double rnorm = c.norm;
+        int grp = -1, ngroups = 0;
+        // settles the pooled pairs of the queries in `filter` (pass 0: all)
+        auto settle_pass = [&](unsigned long long filter, bool first) {
+            // 2. settle: lane l = pair (r, query l); every valid lane ends with
+            // its pair's output in out_*
+            const int used = uniform(*pool_n);
+            const int n = cnt[lane];
+            const bool mine = valid && ((filter >> lane) & 1ull);
+            const bool dropped = pdrop[lane] != 0;
+            if (first) {
+                if (mine && n > kMidCap) queue_heavy(n);
+                const bool pending = mine && dropped && n <= kMidCap;
+                int x = pending ? n : 0, incl = x;
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const int y = __shfl_up(incl, o);
+                    if (lane >= o) incl += y;
+                }
+                grp = pending ? (incl - x) / kMidCap : -1;
+                int top = grp + 1;
+                for (int o = kWave / 2; o > 0; o >>= 1) {
+                    const int y = __shfl_xor(top, o);
+                    top = y > top ? y : top;
+                }
+                ngroups = uniform(top);
+            } else if (mine && dropped) {  // cannot happen (a group fits); stay exact
+                queue_heavy(n);
+            }
+            const bool settle = mine && !dropped && n <= kMidCap;
+            // 2a. up to K candidates: walk the pair's list into registers
+            const bool small = settle && n <= K;
+            int bound = small ? n : 0;  // wave-wide max of the small counts
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const int o2 = __shfl_xor(bound, off);
+                bound = o2 > bound ? o2 : bound;
+            }
+            bound = uniform(bound);  // scalar: the unrolled steps past it are branched over
+            LaneCandidates<K> cand;
+            cand.reset();
+            int p = small ? head[lane] : -1;
+            while (__ballot(p >= 0)) {
+                if (p >= 0) {
+                    cand.insert(pv[p], pij[p], bound);
+                    p = pnext[p];
+                }
+            }
+            if (small) {
+                double sc = 0.0;
+                int m = 0;
+                cand.greedy(sc, m, bound);
+                out_score = n ? (float)(sc / (rnorm * qnorm_l[lane])) : 0.0f;
+                out_matches = m;
+                out_overflow = n >= a.match_limit;
+            }
+            // 2b. K < n <= 64: the whole wave settles the pair from registers
+            unsigned long long mid = __ballot(settle && n > K);
+            while (mid) {
+                const int sl = __builtin_ctzll(mid);
+                mid &= mid - 1ull;
+                int have = 0;
+                const int lim = used < kPool ? used : kPool;
+                for (int base = 0; base < lim; base += kWave) {  // compact the pair's entries
+                    const int e = base + lane;
+                    const bool mine_e = e < lim && pq[e] == sl;
+                    const unsigned long long mk = __ballot(mine_e);
+                    // at most kMidCap entries belong to sl, so every slot is < 64
+                    if (mine_e) midx[have + __popcll(mk & ((1ull << lane) - 1ull))] = e;
+                    have += __popcll(mk);
+                }
+                wave_sync();
+                double v = -__builtin_huge_val();
+                uint32_t ij = 0u;
+                if (lane < have) {
+                    const int e = midx[lane];
+                    v = pv[e];
+                    ij = pij[e];
+                }
+                wave_sync();
+                wave_sort_desc(v, ij);
+                const int vi = (int)(ij >> 16), vj = (int)(ij & 0xffffu);
+                bool alive = lane < have;
+                double sc = 0.0;  // the same on every lane (read from lane f)
+                int m = 0;
+                unsigned long long mk = __ballot(alive);
+                while (mk) {
+                    const int f = __builtin_ctzll(mk);
+                    const int fi = lane_int(vi, f), fj = lane_int(vj, f);
+                    sc += lane_double(v, f);
+                    ++m;
+                    alive = alive & (lane != f) & (vi != fi) & (vj != fj);
+                    mk = __ballot(alive);
+                }
+                if (lane == sl) {
+                    out_score = (float)(sc / (rnorm * qnorm_l[sl]));
+                    out_matches = m;
+                    out_overflow = n >= a.match_limit;
+                }
+            }
+            wave_sync();
+            cnt[lane] = 0;
+            head[lane] = -1;
+            pdrop[lane] = 0;
+            if (lane == 0) *pool_n = 0;
+            wave_sync();
+        };
+        if (tile_f32)
+            probe(std::true_type{}, std::true_type{}, ~0ull);
+        else
+            probe(std::false_type{}, std::true_type{}, ~0ull);
+        wave_sync();
+        flush();  // the previous reference's outputs
+        out_row = r;
+        settle_pass(~0ull, true);
+        // later passes: one group of pending queries each, reference peaks
+        // read again from memory
+        for (int g = 0; g < ngroups; ++g) {
+            const unsigned long long filter = __ballot(grp == g);
+            if (!filter) continue;
+            if (tile_f32)
+                probe(std::true_type{}, std::false_type{}, filter);
+            else
+                probe(std::false_type{}, std::false_type{}, filter);
+            wave_sync();
+            settle_pass(filter, false);
         }
-        wave_sync();
-        cnt[lane] = 0;
-        head[lane] = -1;
-        pdrop[lane] = 0;
-        if (lane == 0) *pool_n = 0;
-        wave_sync();
     };
 
     // One wait per step, before the next prefetch is issued: what is then in

@@ -93,11 +93,15 @@ def pack_bits(fp: np.ndarray) -> np.ndarray:
 
 @dataclass
 class Job:
+    """One matrix of a batch (the fields cudams.batch.CosineJob has)."""
+
     refs: PackedSpectra
     queries: PackedSpectra
     tolerance: float
     mz_power: float
     intensity_power: float
+    shift: float = 0.0
+    match_limit: int = 2048
 
 
 def batched_jobs(seed: int = 6, n_jobs: int = 64, r_range=(16, 2048), q_range=(16, 4096),
