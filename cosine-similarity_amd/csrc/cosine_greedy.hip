@@ -39,7 +39,6 @@ constexpr int kWave = 64;
 constexpr int kLightWaves = 16;
 constexpr int kLightThreads = kWave * kLightWaves;
 constexpr int kMaxLightRefs = 1024;   // largest refs_per_block
-constexpr int kTileLdsCap = 15000;    // sorted tile peaks staged in LDS (larger: read from HBM)
 constexpr int kDefaultK = 8;
 constexpr int kDefaultRound = 256;
 constexpr int kHeavyGrid = 4096;
@@ -469,6 +468,11 @@ constexpr size_t light_lds_bytes(int cap) {
            + (size_t)(cap + 1 + 15) / 16 * 16;      // query lane (u8)
 }
 constexpr int kLightPeakBytes = 4 + 4 + 2 + 1;
+// LDS of one gfx950 workgroup (and CU): the lane kernel is laid out for it at
+// compile time, so every LDS array sits at a constant offset.
+constexpr size_t kLdsBytes = 160 * 1024;
+constexpr int kLightCap = (int)((kLdsBytes - light_lds_bytes(0) - 64) / kLightPeakBytes) - 1;
+static_assert(light_lds_bytes(kLightCap) <= kLdsBytes, "lane kernel LDS fits a workgroup");
 
 // Descending bitonic sort of one key per lane across the wave (64 lanes).
 __device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
@@ -525,7 +529,7 @@ template <int K, bool F32>
 __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     extern __shared__ __align__(16) unsigned char light_lds[];
     __shared__ int next_ref;
-    const int cap = a.lds_cap;
+    constexpr int cap = kLightCap;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     // LDS layout (see light_lds_bytes)
     int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
@@ -761,30 +765,36 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             if (first) {
                 if (mine && n > kMidCap) queue_heavy(n);
                 const bool pending = mine && dropped && n <= kMidCap;
-                int x = pending ? n : 0, incl = x;
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const int y = __shfl_up(incl, o);
-                    if (lane >= o) incl += y;
+                if (__ballot(pending)) {  // rare: only after the pool overflowed
+                    int x = pending ? n : 0, incl = x;
+                    for (int o = 1; o < kWave; o <<= 1) {
+                        const int y = __shfl_up(incl, o);
+                        if (lane >= o) incl += y;
+                    }
+                    grp = pending ? (incl - x) / kMidCap : -1;
+                    int top = grp + 1;
+                    for (int o = kWave / 2; o > 0; o >>= 1) {
+                        const int y = __shfl_xor(top, o);
+                        top = y > top ? y : top;
+                    }
+                    ngroups = uniform(top);
                 }
-                grp = pending ? (incl - x) / kMidCap : -1;
-                int top = grp + 1;
-                for (int o = kWave / 2; o > 0; o >>= 1) {
-                    const int y = __shfl_xor(top, o);
-                    top = y > top ? y : top;
-                }
-                ngroups = uniform(top);
             } else if (mine && dropped) {  // cannot happen (a group fits); stay exact
                 queue_heavy(n);
             }
             const bool settle = mine && !dropped && n <= kMidCap;
             // 2a. up to K candidates: walk the pair's list into registers
             const bool small = settle && n <= K;
-            int bound = small ? n : 0;  // wave-wide max of the small counts
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const int o2 = __shfl_xor(bound, off);
-                bound = o2 > bound ? o2 : bound;
+            // wave-wide max of the small counts, by ballots (no LDS round trip);
+            // a scalar, so the unrolled steps past it are branched over
+            int bound = 0;
+#pragma unroll
+            for (int b = K; b > 0; --b) {
+                if (__ballot(small && n >= b)) {
+                    bound = b;
+                    break;
+                }
             }
-            bound = uniform(bound);  // scalar: the unrolled steps past it are branched over
             LaneCandidates<K> cand;
             cand.reset();
             int p = small ? head[lane] : -1;
@@ -1000,7 +1010,7 @@ __device__ void greedy_sorted(const double *bv, const uint32_t *bij, int n, uint
         const uint32_t ij = valid ? bij[idx] : 0u;
         const double v = valid ? bv[idx] : 0.0;
         const int i = (int)(ij >> 16), j = (int)(ij & 0xffffu);
-        bool alive = valid & !bit_set(used_r, i) & !bit_set(used_q, j);
+        bool alive = (int)valid & (int)!bit_set(used_r, i) & (int)!bit_set(used_q, j);
         bool acc = false;
         unsigned long long mask = __ballot(alive);
         while (mask) {
@@ -1439,25 +1449,27 @@ cs_status check_spectra(const cs_spectra *s, const char *name) {
     return CS_OK;
 }
 
-// Largest tile capacity whose k_light<K> LDS fits the device limit (cached).
+// Tile capacity of k_light (kLightCap), or 0 when the device offers less LDS
+// per workgroup than the kernel's compile-time layout needs.
 int light_capacity(int K) {
+    (void)K;
     static int max_lds = -1;
     if (max_lds < 0) {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            v = 65536;
+            v = 0;
         max_lds = v;
     }
-    const size_t fixed = light_lds_bytes(0) + 64;  // + alignment slack
-    const size_t room = (size_t)max_lds > fixed ? (size_t)max_lds - fixed : 0;
-    const int cap = (int)(room / kLightPeakBytes) - 1;
-    return cap < kTileLdsCap ? (cap > 0 ? cap : 0) : kTileLdsCap;
+    return (size_t)max_lds >= light_lds_bytes(kLightCap) ? kLightCap : 0;
 }
 
 template <int K, bool F32>
 cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
     a.lds_cap = light_capacity(K);
+    if (a.lds_cap == 0)
+        return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
+                    light_lds_bytes(kLightCap));
     const size_t lds = light_lds_bytes(a.lds_cap);
     static bool attr_set = false;
     if (!attr_set) {
@@ -1631,14 +1643,8 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
     int K = params->light_candidates ? params->light_candidates : kDefaultK;
     const int T = params->round_capacity ? params->round_capacity : kDefaultRound;
     hipStream_t hs = static_cast<hipStream_t>(stream);
-    const SpecView R = view(Rs), Q = view(Qs);
-    const float mzmax = Rs.max_abs_mz > Qs.max_abs_mz ? Rs.max_abs_mz : Qs.max_abs_mz;
-    const float tolc = coarse_tolerance(tol, shift, mzmax);
+    const SpecView Q = view(Qs);
     const float shift32 = (float)shift;
-    auto blocks = [](int64_t n, int per) {
-        int64_t b = (n + per - 1) / per;
-        return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
-    };
     const int stages = params->stages ? params->stages
                                       : (CS_STAGE_PREPARE | CS_STAGE_LIGHT | CS_STAGE_HEAVY);
     if (nQ == 0) return CS_OK;
@@ -1978,6 +1984,11 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
     b->bytes = b->off_scratch + need.scratch_bytes;
     // LDS attributes now, so that a run can be captured into a graph
     b->cap = light_capacity(b->K);
+    if (b->cap == 0) {
+        delete b;
+        return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
+                    light_lds_bytes(kLightCap));
+    }
     static size_t heavy_attr_b = 0, giant_attr_b = 0;
     if ((st = b->f32 ? prime_light_batch_k<true>(b->K, b->cap)
                      : prime_light_batch_k<false>(b->K, b->cap)) != CS_OK ||
