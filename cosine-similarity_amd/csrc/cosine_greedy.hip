@@ -445,15 +445,19 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Per-wave candidate pool of k_light: every exact candidate of the current
-// reference against the tile, linked per query (LIFO; order is irrelevant,
-// the settle step sorts).  A query contributes at most kMidCap entries (more
-// means the pair is settled by the heavy kernel anyway).
+// reference against the tile, in emit order, with its query lane (pq).  The
+// first kIdxRanks candidates of each query are also indexed by their rank
+// (pidx[rank][query], the rank is what the query's count atomic returns), so
+// a lane gathers its pair's candidates with independent loads.  A query
+// contributes at most kMidCap entries (more means the pair is settled by the
+// heavy kernel anyway).
 constexpr int kPool = 128;
+constexpr int kIdxRanks = 16;   // >= the largest light_candidates
 constexpr int kMidCap = 64;     // candidates a wave settles for one pair in registers
 constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first
 constexpr size_t kWaveLds =
-    ((size_t)kPool * (8 + 4 + 4 + 1) + (size_t)kTileQueries * (4 * 2 + 1) + 16 + kWave * 4 + 15) /
-    16 * 16;
+    ((size_t)kPool * (8 + 4 + 1) + (size_t)kTileQueries * (4 + 1 + kIdxRanks) + 16 + kWave * 4 +
+     15) / 16 * 16;
 constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
 
 // Bytes of dynamic LDS of k_light for a tile capacity `cap`.
@@ -539,13 +543,12 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     unsigned char *wbase = pools + wave * kWaveLds;
     double *pv = reinterpret_cast<double *>(wbase);
     uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
-    int *pnext = reinterpret_cast<int *>(pij + kPool);
-    int *cnt = pnext + kPool;
-    int *head = cnt + kTileQueries;
-    int *pool_n = head + kTileQueries;
+    int *cnt = reinterpret_cast<int *>(pij + kPool);
+    int *pool_n = cnt + kTileQueries;
     int *midx = pool_n + 4;
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
     unsigned char *pdrop = pq + kPool;  // query lost a candidate to a full pool
+    unsigned char *pidx = pdrop + kTileQueries;  // [kIdxRanks][kTileQueries] pool slots
     float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * kWaveLds);
     float *sint_l = smz_l + cap + 1;
     uint16_t *dir_l = reinterpret_cast<uint16_t *>(sint_l + cap + 1);
@@ -609,7 +612,6 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     }
     for (int x = tid; x <= nrefs; x += kLightThreads) roff_l[x] = (int32_t)(R.off[r0 + x] - rbase);
     cnt[lane] = 0;
-    head[lane] = -1;
     pdrop[lane] = 0;
     if (lane == 0) *pool_n = 0;
     if (tid == 0) next_ref = 0;
@@ -686,8 +688,13 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                         mr = R.mzx ? R.mzx[roff + i] : (double)ri;
                     }
                     if (!exact_window(mr, mq, a.tol, a.shift)) return;
-                    if (atomicAdd(&cnt[ql], 1) >= kMidCap) return;  // heavy anyway
+                    // rank within the query and pool slot, both atomics in flight at once
+                    const int rank = atomicAdd(&cnt[ql], 1);
                     const int pos = atomicAdd(pool_n, 1);
+                    if (rank >= kMidCap) {  // heavy anyway: the slot holds no candidate
+                        if (pos < kPool) pq[pos] = 0xffu;
+                        return;
+                    }
                     if (pos >= kPool) {
                         pdrop[ql] = 1;
                         return;
@@ -700,7 +707,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                         pv[pos] = pair_value(R, roff + i, Q, qoff_l[ql] + j);
                     pij[pos] = pack_ij(i, j);
                     pq[pos] = (unsigned char)ql;
-                    pnext[pos] = atomicExch(&head[ql], pos);
+                    if (rank < kIdxRanks) pidx[rank * kTileQueries + ql] = (unsigned char)pos;
                 };
                 if (in_lds) {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return smz_l[p]; },
@@ -795,13 +802,15 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     break;
                 }
             }
+            static_assert(K <= kIdxRanks, "every small pair's candidates are indexed by rank");
             LaneCandidates<K> cand;
             cand.reset();
-            int p = small ? head[lane] : -1;
-            while (__ballot(p >= 0)) {
-                if (p >= 0) {
-                    cand.insert(pv[p], pij[p], bound);
-                    p = pnext[p];
+#pragma unroll
+            for (int s = 0; s < K; ++s) {  // independent gathers through the rank index
+                if (s >= bound) break;     // uniform
+                if (small && s < n) {
+                    const int e = pidx[s * kTileQueries + lane];
+                    cand.insert(pv[e], pij[e], bound);
                 }
             }
             if (small) {
@@ -858,7 +867,6 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             }
             wave_sync();
             cnt[lane] = 0;
-            head[lane] = -1;
             pdrop[lane] = 0;
             if (lane == 0) *pool_n = 0;
             wave_sync();
