@@ -974,6 +974,9 @@ __device__ __forceinline__ bool bit_set(const uint32_t *b, int x) {
 
 // Pad [n, next_pow2(n)) with -inf keys and sort the first next_pow2(n)
 // entries descending by key_greater, with every thread of the block.
+// Thread t takes compare-exchange pairs t, t + nt, ...; a wave's 64
+// consecutive pairs always span one aligned block of 128 keys, so stages of
+// stride <= 64 stay inside the wave and need no workgroup barrier between them.
 __device__ void sort_desc(double *bv, uint32_t *bij, int n) {
     const int tid = threadIdx.x, nt = blockDim.x;
     int n2 = 1;
@@ -999,9 +1002,14 @@ __device__ void sort_desc(double *bv, uint32_t *bij, int n) {
                     bij[hi] = ia;
                 }
             }
-            __syncthreads();
+            const int next = stride > 1 ? stride >> 1 : size;  // the next stage's stride
+            if (stride > kWave || next > kWave)
+                __syncthreads();
+            else
+                wave_sync();
         }
     }
+    __syncthreads();
 }
 
 // Greedy over sorted entries [0, n), run by one wave: chunks of 64 in order;
