@@ -27,7 +27,7 @@ sys.path.insert(0, REPO)
 sys.path.insert(0, os.path.join(REPO, "cosine-similarity_amd"))
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
-INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA
+FP4_PEAK_TFLOPS = 10100.0  # MI355X dense fp4 MFMA
 
 
 def log(msg: str) -> None:
@@ -195,10 +195,10 @@ def bench_fingerprint(a) -> dict:
         "metric": "FingerprintSimilarity jaccard pairs/s at 4096 x 4096 x 2048 bits",
         "value": n * n / (ms / 1e3), "unit": "pairs/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int8", "data": "synthetic",
+        "vs_baseline": None, "dtype": "fp4", "data": "synthetic",
         "config": {"workload": "fingerprint_4096x4096x2048_jaccard"},
-        "roofline": {"bound": "mfma", "achieved": ops / (ms / 1e3) / 1e12, "peak": INT8_PEAK_TOPS,
-                     "unit": "TFLOP/s", "frac": ops / (ms / 1e3) / 1e12 / INT8_PEAK_TOPS,
+        "roofline": {"bound": "mfma", "achieved": ops / (ms / 1e3) / 1e12, "peak": FP4_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": ops / (ms / 1e3) / 1e12 / FP4_PEAK_TFLOPS,
                      "traffic": None},
         "cpu_baseline": None,
     }

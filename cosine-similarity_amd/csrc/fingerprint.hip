@@ -1,181 +1,341 @@
 // fingerprint.hip -- Jaccard / Dice / cosine between binary fingerprints on
-// gfx950 int8 matrix cores.
+// gfx950 matrix cores.
 //
 // Replaces the torch float32 GEMM + elementwise epilogues of
 // cudams/similarity/vector_similarity_functions.py:12-47 (jaccard),
 // :50-91 (dice) and :94-119 (cosine).
 //
-// Pipeline (asynchronous on the caller's stream):
-//   1. k_fp_expand   bit-packed uint32 rows -> int8 0/1 rows, zero padded to a
-//                    multiple of 128 rows and 64 bytes of K
-//   2. k_fp_popcount |row| per fingerprint (exact int32)
-//   3. k_fp_gemm     128 x 128 output tile per 256-thread workgroup, 2 x 2
-//                    waves of 64 x 64, v_mfma_i32_16x16x64_i8 (exact int32
-//                    intersections), K staged through LDS in 64-byte slices,
-//                    measure + missing-fingerprint fill fused in the epilogue.
+// One kernel, no workspace.  The packed uint32 rows are read as they are;
+// each 4-bit nibble plane of a word becomes 8 fp4 (e2m1) values with one
+// VALU op:
+//     plane 0: w & 0x11111111 -> 0.5     plane 1: w & 0x22222222 -> 1.0
+//     plane 2: w & 0x44444444 -> 2.0     plane 3: (w >> 1) & 0x44444444 -> 2.0
+// (a cleared bit gives 0).  v_mfma_scale_f32_16x16x128_f8f6f4 multiplies
+// both operands by a power-of-two block scale (2, 1, 1/2, 1/2 for the four
+// planes), so every set bit is exactly 1.0 and the float32 accumulator holds
+// the exact intersection count (integers below 2^24).  Reference and query
+// operands use the same bit -> element mapping, so the element order the
+// matrix core assigns to K does not matter.
+//
+// Workgroup: 256 threads, 2 x 2 waves, each wave a 64-reference x 64-query
+// tile = 4 x 4 blocks of 16 x 16; two workgroups per CU (amdgpu_waves_per_eu
+// 2, no LDS).  The grid is persistent (2 workgroups per CU) and walks the
+// 128 x 128 block tiles.  Each lane loads, for each of its 8 fragments (4
+// reference, 4 query rows), 4 words (128 bits) per 512-bit step straight
+// from global memory (the packed rows are small and stay in L2), one step
+// ahead; the last step of a tile prefetches the next tile's first step.  Row
+// popcounts come from v_bcnt on the same words.  The epilogue writes 4
+// consecutive queries per lane (one 16-byte non-temporal store) with the
+// measure computed in float32 (jaccard, dice: the correctly rounded quotient
+// of exact integers, which equals the float32 rounding of the float64
+// quotient) or float64 (cosine).
+//
+// On gfx950 VALU instructions and MFMAs of one SIMD do not overlap, so the
+// kernel is priced as MFMA cycles + VALU cycles: every VALU op in the K loop
+// (plane masks, popcounts) and in the epilogue (the division) counts.
 #include <limits.h>
+
+#include <algorithm>
+#include <atomic>
 
 #include "cs_internal.h"
 
 namespace cs {
 namespace {
 
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int kTile = 128;
-constexpr int kKc = 64;          // bytes of K per MFMA step
+#ifndef CS_FP_REF_FRAGS
+#define CS_FP_REF_FRAGS 4
+#endif
+#ifndef CS_FP_QUERY_FRAGS
+#define CS_FP_QUERY_FRAGS 4
+#endif
+constexpr int kRF = CS_FP_REF_FRAGS;     // 16-reference fragments per wave
+constexpr int kQF = CS_FP_QUERY_FRAGS;   // 16-query fragments per wave
+constexpr int kRefTile = 2 * 16 * kRF;   // references per workgroup (2 x 2 waves)
+constexpr int kQueryTile = 2 * 16 * kQF;
 constexpr int kThreads = 256;
-constexpr int kRowStride = 80;   // LDS bytes per row: 64 + 16 pad keeps ds_read_b128 conflict-free
+constexpr int kStepWords = 16;   // 512 bits of K per step (4 lane groups x 4 words)
+constexpr int64_t kMaxBits = 1 << 23;  // counts and |a| + |b| stay exact in float32
 
-__global__ void k_fp_expand(const uint32_t *__restrict__ words, int64_t rows, int64_t rows_pad,
-                            int64_t n_words, int64_t k_bytes, uint32_t *__restrict__ out) {
-    // out row x holds k_bytes bytes = k_bytes / 4 dwords; each thread writes one dword
-    const int64_t dw_per_row = k_bytes / 4;
-    const int64_t total = rows_pad * dw_per_row;
-    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t x = e / dw_per_row;
-        const int64_t d = e - x * dw_per_row;  // bytes 4d .. 4d+3 = bits 4d .. 4d+3
-        const int64_t w = d >> 3;              // 8 dwords per source word
-        uint32_t v = 0u;
-        if (x < rows && w < n_words) {
-            const uint32_t nib = (words[x * n_words + w] >> ((d & 7) * 4)) & 0xfu;
-            v = (nib * 0x00204081u) & 0x01010101u;  // bit b of the nibble -> byte b
-        }
-        out[e] = v;
+template <bool kFull>
+__device__ __forceinline__ v4u load_words(const uint32_t *__restrict__ row, int64_t w,
+                                          int64_t n_words) {
+    if (kFull) return *reinterpret_cast<const v4u *>(row + w);
+    v4u v;
+    v.x = w < n_words ? row[w] : 0u;
+    v.y = w + 1 < n_words ? row[w + 1] : 0u;
+    v.z = w + 2 < n_words ? row[w + 2] : 0u;
+    v.w = w + 3 < n_words ? row[w + 3] : 0u;
+    return v;
+}
+
+template <bool kFull>
+__device__ __forceinline__ void load_step(const uint32_t *const (&rrow)[kRF],
+                                          const uint32_t *const (&qrow)[kQF], int64_t w,
+                                          int64_t n_words, v4u (&r)[kRF], v4u (&q)[kQF]) {
+#pragma unroll
+    for (int f = 0; f < kRF; ++f) r[f] = load_words<kFull>(rrow[f], w, n_words);
+#pragma unroll
+    for (int f = 0; f < kQF; ++f) q[f] = load_words<kFull>(qrow[f], w, n_words);
+}
+
+template <int P>
+__device__ __forceinline__ v8i plane(v4u x) {
+    v4u y;
+    if (P == 0) y = x & 0x11111111u;
+    else if (P == 1) y = x & 0x22222222u;
+    else if (P == 2) y = x & 0x44444444u;
+    else y = (x >> 1u) & 0x44444444u;
+    return v8i{(int)y.x, (int)y.y, (int)y.z, (int)y.w, 0, 0, 0, 0};
+}
+
+// E8M0 block scale that turns plane P's nonzero value into 1.0 (the same
+// scale on both operands: (v * s)^2 = 1 for v * s = 1).
+template <int P>
+constexpr int plane_scale() { return P == 0 ? 128 : (P == 1 ? 127 : 126); }
+
+// acc + popcount(x) as one v_bcnt_u32_b32 per word (the compiler otherwise
+// splits the chain into separate counts and adds).
+__device__ __forceinline__ uint32_t popc4(v4u x, uint32_t acc) {
+    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(x.x));
+    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(x.y));
+    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(x.z));
+    asm("v_bcnt_u32_b32 %0, %1, %0" : "+v"(acc) : "v"(x.w));
+    return acc;
+}
+
+
+// Correctly rounded n / d for integers 0 <= n <= 2^24, 1 <= d <= 2^24 held
+// exactly in float32.  This is the IEEE float32 division sequence (reciprocal,
+// one Newton step on it, two residual corrections of the quotient, the last
+// FMA rounding once) without the operand scaling and special-case fixup,
+// which are identities on this domain: no operand or intermediate is
+// denormal, infinite or NaN, and n = 0 gives +0 exactly.
+__device__ __forceinline__ float div_rn(float n, float d) {
+    float r = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, r, 1.0f);
+    r = fmaf(e, r, r);
+    float q = n * r;
+    float t = fmaf(-d, q, n);
+    q = fmaf(t, r, q);
+    t = fmaf(-d, q, n);
+    return fmaf(t, r, q);
+}
+
+// Measure from the exact counts (float32 integers up to 2^23 bits, so every
+// sum below is exact).  A zero denominator only occurs with inter = 0, where
+// dividing by 1 gives the reference's nan_to_num 0.
+template <int kMeasure>
+__device__ __forceinline__ float measure_value(float inter, float a, float b) {
+    if (kMeasure == CS_JACCARD) return div_rn(inter, fmaxf((a - inter) + b, 1.0f));
+    if (kMeasure == CS_DICE) return 2.0f * div_rn(inter, fmaxf(a + b, 1.0f));
+    return (float)((double)inter / sqrt(fmax((double)a * (double)b, 1.0)));
+}
+
+template <int P>
+__device__ __forceinline__ void plane_mfma(const v4u (&r)[kRF], const v4u (&q)[kQF],
+                                           v4f (&acc)[kQF][kRF]) {
+    v8i br[kRF];
+#pragma unroll
+    for (int f = 0; f < kRF; ++f) br[f] = plane<P>(r[f]);
+#pragma unroll
+    for (int qf = 0; qf < kQF; ++qf) {
+        const v8i aq = plane<P>(q[qf]);
+#pragma unroll
+        for (int rf = 0; rf < kRF; ++rf)
+            acc[qf][rf] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                aq, br[rf], acc[qf][rf], 4, 4, 0, plane_scale<P>(), 0, plane_scale<P>());
     }
 }
 
-__global__ void k_fp_popcount(const uint32_t *__restrict__ words, int64_t rows, int64_t n_words,
-                              int32_t *__restrict__ pop) {
-    for (int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; x < rows;
-         x += (int64_t)gridDim.x * blockDim.x) {
-        int32_t c = 0;
-        for (int64_t w = 0; w < n_words; ++w) c += __popc(words[x * n_words + w]);
-        pop[x] = c;
+// One 512-bit step: popcounts, then the 4 planes x kQF x kRF blocks.
+__device__ __forceinline__ void fp_step(const v4u (&r)[kRF], const v4u (&q)[kQF],
+                                        v4f (&acc)[kQF][kRF], uint32_t (&pr)[kRF],
+                                        uint32_t (&pq)[kQF]) {
+#pragma unroll
+    for (int f = 0; f < kRF; ++f) pr[f] = popc4(r[f], pr[f]);
+#pragma unroll
+    for (int f = 0; f < kQF; ++f) pq[f] = popc4(q[f], pq[f]);
+    // The barriers keep the scheduler from expanding every plane up front
+    // (32 extra registers per plane).
+    plane_mfma<0>(r, q, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    plane_mfma<1>(r, q, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    plane_mfma<2>(r, q, acc);
+    __builtin_amdgcn_sched_barrier(0);
+    plane_mfma<3>(r, q, acc);
+}
+
+// Wave-level row bases of block tile `tile` and the row pointers the lane
+// loads from.  Rows past the end are clamped to the last row (computed,
+// never stored).
+__device__ __forceinline__ void set_tile(int64_t tile, int64_t gx, int wave, int lr,
+                                         const uint32_t *R, int64_t nR, const uint32_t *Q,
+                                         int64_t nQ, int64_t n_words, int64_t &rb, int64_t &qb,
+                                         const uint32_t *(&rrow)[kRF],
+                                         const uint32_t *(&qrow)[kQF]) {
+    const int64_t ty = tile / gx, tx = tile - ty * gx;
+    rb = ty * kRefTile + (wave >> 1) * (16 * kRF);
+    qb = tx * kQueryTile + (wave & 1) * (16 * kQF);
+#pragma unroll
+    for (int f = 0; f < kRF; ++f) {
+        const int64_t r = rb + 16 * f + lr;
+        rrow[f] = R + (r < nR ? r : nR - 1) * n_words;
+    }
+#pragma unroll
+    for (int f = 0; f < kQF; ++f) {
+        const int64_t q = qb + 16 * f + lr;
+        qrow[f] = Q + (q < nQ ? q : nQ - 1) * n_words;
     }
 }
 
-__device__ __forceinline__ float measure_value(int measure, int32_t inter, int32_t a, int32_t b) {
-    if (measure == CS_JACCARD) {
-        const int32_t uni = a + b - inter;
-        return uni ? (float)((double)inter / (double)uni) : 0.0f;
+// Measures of one wave tile from its accumulators and lane popcounts.
+template <int kMeasure>
+__device__ __forceinline__ void fp_epilogue(const v4f (&acc)[kQF][kRF], uint32_t (&pr)[kRF],
+                                            uint32_t (&pq)[kQF], int64_t rb, int64_t qb,
+                                            int lr, int grp, int64_t nR, int64_t nQ,
+                                            const uint8_t *__restrict__ ref_valid,
+                                            const uint8_t *__restrict__ query_valid,
+                                            float empty_score, float *__restrict__ out,
+                                            int32_t *__restrict__ inter_out, int64_t ld,
+                                            int vec_store) {
+    // Row popcounts: sum the 4 lane groups; lane (lr, *) then holds the count
+    // of row 16 f + lr of each fragment.
+#pragma unroll
+    for (int f = 0; f < kRF; ++f) {
+        pr[f] += __shfl_xor(pr[f], 16);
+        pr[f] += __shfl_xor(pr[f], 32);
     }
-    if (measure == CS_DICE) {
-        const int32_t den = a + b;
-        return den ? (float)(2.0 * ((double)inter / (double)den)) : 0.0f;
+#pragma unroll
+    for (int f = 0; f < kQF; ++f) {
+        pq[f] += __shfl_xor(pq[f], 16);
+        pq[f] += __shfl_xor(pq[f], 32);
     }
-    return (a && b) ? (float)((double)inter / sqrt((double)a * (double)b)) : 0.0f;
-}
+    // Accumulator layout (16 x 16 f32): lane (lr, grp) element e holds
+    // query 4 grp + e (operand A row), reference lr (operand B row).
+    float qcount[kQF][4];
+#pragma unroll
+    for (int f = 0; f < kQF; ++f)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) qcount[f][e] = (float)__shfl(pq[f], 4 * grp + e);
+    const bool flagged = ref_valid || query_valid;
 
-__global__ void __launch_bounds__(kThreads)
-k_fp_gemm(const int8_t *__restrict__ A, const int8_t *__restrict__ B, int64_t k_bytes,
-          const int32_t *__restrict__ pa, const int32_t *__restrict__ pb, int64_t nR, int64_t nQ,
-          const uint8_t *__restrict__ ref_valid, const uint8_t *__restrict__ query_valid,
-          float empty_score, int measure, float *__restrict__ out,
-          int32_t *__restrict__ inter_out, int64_t ld) {
-    __shared__ __align__(16) int8_t lA[kTile * kRowStride];
-    __shared__ __align__(16) int8_t lB[kTile * kRowStride];
-
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int64_t r0 = (int64_t)blockIdx.y * kTile;
-    const int64_t q0 = (int64_t)blockIdx.x * kTile;
-
-    v4i acc[4][4];
 #pragma unroll
-    for (int bi = 0; bi < 4; ++bi)
+    for (int rf = 0; rf < kRF; ++rf) {
+        const int64_t r = rb + 16 * rf + lr;
+        if (r >= nR) continue;
+        const float a = (float)pr[rf];
+        const bool rv = !ref_valid || ref_valid[r];
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj) acc[bi][bj] = v4i{0, 0, 0, 0};
-
-    // Global -> LDS mapping: 128 rows x 64 bytes = 512 chunks of 16 bytes per
-    // matrix; thread tid moves chunks tid and tid + 256 of A and of B.
-    const int c0 = tid, c1 = tid + kThreads;
-    const int row0 = c0 >> 2, part0 = c0 & 3, row1 = c1 >> 2, part1 = c1 & 3;
-    const int8_t *a0 = A + (r0 + row0) * k_bytes + 16 * part0;
-    const int8_t *a1 = A + (r0 + row1) * k_bytes + 16 * part1;
-    const int8_t *b0 = B + (q0 + row0) * k_bytes + 16 * part0;
-    const int8_t *b1 = B + (q0 + row1) * k_bytes + 16 * part1;
-
-    v4i ga0 = *reinterpret_cast<const v4i *>(a0);
-    v4i ga1 = *reinterpret_cast<const v4i *>(a1);
-    v4i gb0 = *reinterpret_cast<const v4i *>(b0);
-    v4i gb1 = *reinterpret_cast<const v4i *>(b1);
-
-    const int frag_row = lane & 15, frag_k = 16 * (lane >> 4);
-    for (int64_t kc = 0; kc < k_bytes; kc += kKc) {
-        *reinterpret_cast<v4i *>(&lA[row0 * kRowStride + 16 * part0]) = ga0;
-        *reinterpret_cast<v4i *>(&lA[row1 * kRowStride + 16 * part1]) = ga1;
-        *reinterpret_cast<v4i *>(&lB[row0 * kRowStride + 16 * part0]) = gb0;
-        *reinterpret_cast<v4i *>(&lB[row1 * kRowStride + 16 * part1]) = gb1;
-        __syncthreads();
-        if (kc + kKc < k_bytes) {  // prefetch the next slice while the MFMAs run
-            ga0 = *reinterpret_cast<const v4i *>(a0 + kc + kKc);
-            ga1 = *reinterpret_cast<const v4i *>(a1 + kc + kKc);
-            gb0 = *reinterpret_cast<const v4i *>(b0 + kc + kKc);
-            gb1 = *reinterpret_cast<const v4i *>(b1 + kc + kKc);
-        }
-        v4i fa[4], fb[4];
+        for (int qf = 0; qf < kQF; ++qf) {
+            const int64_t q = qb + 16 * qf + 4 * grp;
+            v4f v;
 #pragma unroll
-        for (int bi = 0; bi < 4; ++bi)
-            fa[bi] = *reinterpret_cast<const v4i *>(
-                &lA[(wr * 64 + bi * 16 + frag_row) * kRowStride + frag_k]);
+            for (int e = 0; e < 4; ++e)
+                v[e] = measure_value<kMeasure>(acc[qf][rf][e], a, qcount[qf][e]);
+            if (flagged) {
 #pragma unroll
-        for (int bj = 0; bj < 4; ++bj)
-            fb[bj] = *reinterpret_cast<const v4i *>(
-                &lB[(wc * 64 + bj * 16 + frag_row) * kRowStride + frag_k]);
+                for (int e = 0; e < 4; ++e) {
+                    const bool qv = !query_valid || (q + e < nQ && query_valid[q + e]);
+                    v[e] = (rv & qv) ? v[e] : empty_score;
+                }
+            }
+            float *o = out + r * ld + q;
+            if (vec_store && q + 3 < nQ) {
+                __builtin_nontemporal_store(v, reinterpret_cast<v4f *>(o));
+                if (inter_out) {
+                    const v4i iv{(int)acc[qf][rf][0], (int)acc[qf][rf][1], (int)acc[qf][rf][2],
+                                 (int)acc[qf][rf][3]};
+                    __builtin_nontemporal_store(iv, reinterpret_cast<v4i *>(inter_out + r * ld + q));
+                }
+            } else {
 #pragma unroll
-        for (int bi = 0; bi < 4; ++bi)
-#pragma unroll
-            for (int bj = 0; bj < 4; ++bj)
-                acc[bi][bj] =
-                    __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[bi], fb[bj], acc[bi][bj], 0, 0, 0);
-        __syncthreads();
-    }
-
-    // Epilogue.  16x16 accumulator layout: lane l, element e holds
-    // row 4 * (l / 16) + e, column l % 16 of the block.
-#pragma unroll
-    for (int bi = 0; bi < 4; ++bi) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t r = r0 + wr * 64 + bi * 16 + 4 * (lane >> 4) + e;
-            if (r >= nR) continue;
-            const int32_t a = pa[r];
-            const bool rv = !ref_valid || ref_valid[r];
-#pragma unroll
-            for (int bj = 0; bj < 4; ++bj) {
-                const int64_t q = q0 + wc * 64 + bj * 16 + (lane & 15);
-                if (q >= nQ) continue;
-                const int32_t inter = acc[bi][bj][e];
-                const bool ok = rv && (!query_valid || query_valid[q]);
-                out[r * ld + q] = ok ? measure_value(measure, inter, a, pb[q]) : empty_score;
-                if (inter_out) inter_out[r * ld + q] = inter;
+                for (int e = 0; e < 4; ++e) {
+                    if (q + e >= nQ) break;
+                    o[e] = v[e];
+                    if (inter_out) inter_out[r * ld + q + e] = (int)acc[qf][rf][e];
+                }
             }
         }
     }
 }
 
-struct FpLayout {
-    int8_t *A, *B;
-    int32_t *pa, *pb;
-    int64_t rows_a, rows_b, k_bytes;
-    size_t bytes;
-};
+// Persistent: workgroup b handles block tiles b, b + gridDim.x, ... (gx
+// tiles per row of tiles).  The first step of the next tile is loaded during
+// the last step of the current one, so its latency hides behind the MFMAs
+// and the epilogue.
+template <bool kFull, int kMeasure>
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_fp_gemm(const uint32_t *__restrict__ R, int64_t nR, const uint32_t *__restrict__ Q, int64_t nQ,
+          int64_t n_words, int64_t gx, int64_t n_tiles, const uint8_t *__restrict__ ref_valid,
+          const uint8_t *__restrict__ query_valid, float empty_score,
+          float *__restrict__ out, int32_t *__restrict__ inter_out, int64_t ld, int vec_store) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lr = lane & 15, grp = lane >> 4;
+    int64_t tile = blockIdx.x;
+    int64_t rb, qb, rb_next = 0, qb_next = 0;
+    const uint32_t *rrow[kRF], *qrow[kQF];
+    set_tile(tile, gx, wave, lr, R, nR, Q, nQ, n_words, rb, qb, rrow, qrow);
 
-FpLayout fp_layout(void *base, int64_t nR, int64_t nQ, int64_t n_words) {
-    FpLayout L{};
-    Carver c(base);
-    L.rows_a = (nR + kTile - 1) / kTile * kTile;
-    L.rows_b = (nQ + kTile - 1) / kTile * kTile;
-    L.k_bytes = (n_words * 32 + kKc - 1) / kKc * kKc;
-    L.A = c.take<int8_t>(L.rows_a * L.k_bytes);
-    L.B = c.take<int8_t>(L.rows_b * L.k_bytes);
-    L.pa = c.take<int32_t>(nR > 0 ? nR : 1);
-    L.pb = c.take<int32_t>(nQ > 0 ? nQ : 1);
-    L.bytes = align_up(c.used, 256);
-    return L;
+    const int steps = (int)((n_words + kStepWords - 1) / kStepWords);
+    v4u r0[kRF], q0[kQF], r1[kRF], q1[kQF];
+    load_step<kFull>(rrow, qrow, 4 * grp, n_words, r0, q0);
+    for (;;) {
+        v4f acc[kQF][kRF];
+#pragma unroll
+        for (int i = 0; i < kQF; ++i)
+#pragma unroll
+            for (int j = 0; j < kRF; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        uint32_t pr[kRF], pq[kQF];
+#pragma unroll
+        for (int f = 0; f < kRF; ++f) pr[f] = 0u;
+#pragma unroll
+        for (int f = 0; f < kQF; ++f) pq[f] = 0u;
+        const int64_t next = tile + gridDim.x;
+        // Two steps per iteration so the prefetch buffers swap roles without
+        // copies.  The full-word path has an even step count (checked on the
+        // host); the bounded path reads zero words past the end (no bits, no
+        // effect).  The last iteration prefetches the next tile's first step.
+        for (int s = 0; s < steps; s += 2) {
+            const int64_t w1 = (int64_t)(s + 1) * kStepWords + 4 * grp;
+            load_step<kFull>(rrow, qrow, w1, n_words, r1, q1);
+            fp_step(r0, q0, acc, pr, pq);
+            if (s + 2 < steps) {
+                load_step<kFull>(rrow, qrow, w1 + kStepWords, n_words, r0, q0);
+            } else if (next < n_tiles) {
+                set_tile(next, gx, wave, lr, R, nR, Q, nQ, n_words, rb_next, qb_next, rrow,
+                         qrow);
+                load_step<kFull>(rrow, qrow, 4 * grp, n_words, r0, q0);
+            }
+            fp_step(r1, q1, acc, pr, pq);
+        }
+        fp_epilogue<kMeasure>(acc, pr, pq, rb, qb, lr, grp, nR, nQ, ref_valid, query_valid,
+                              empty_score, out, inter_out, ld, vec_store);
+        if (next >= n_tiles) break;
+        tile = next;
+        rb = rb_next;
+        qb = qb_next;
+    }
+}
+
+// Compute units of the current device (cached per device; 256 if unknown).
+int device_cu_count() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v < 1)
+        v = 256;
+    cache[dev].store(v, std::memory_order_relaxed);
+    return v;
 }
 
 }  // namespace
@@ -187,7 +347,7 @@ extern "C" cs_status cs_fingerprint_workspace_size(int64_t n_refs, int64_t n_que
                                                    int64_t n_words, size_t *bytes) {
     if (n_refs < 0 || n_queries < 0 || n_words < 0 || !bytes)
         return fail(CS_ERR_INVALID, "invalid fingerprint shape");
-    *bytes = fp_layout(nullptr, n_refs, n_queries, n_words).bytes;
+    *bytes = 0;  // the kernel reads the packed rows directly
     return CS_OK;
 }
 
@@ -199,6 +359,8 @@ extern "C" cs_status cs_fingerprint_similarity(int measure, const uint32_t *ref_
                                                float *out, int32_t *intersections, int64_t ld,
                                                void *workspace, size_t workspace_bytes,
                                                void *stream) {
+    (void)workspace;
+    (void)workspace_bytes;
     if (measure != CS_JACCARD && measure != CS_DICE && measure != CS_COSINE)
         return fail(CS_ERR_INVALID, "unknown measure %d", measure);
     if (n_refs < 0 || n_queries < 0 || n_words < 0)
@@ -207,38 +369,33 @@ extern "C" cs_status cs_fingerprint_similarity(int measure, const uint32_t *ref_
     if (n_words == 0 || !ref_words || !query_words || !out)
         return fail(CS_ERR_INVALID, "fingerprint pointers must not be NULL and n_words > 0");
     if (ld < n_queries) return fail(CS_ERR_INVALID, "ld < number of queries");
-    if (n_words * 32 > (int64_t)INT_MAX / 2)
-        return fail(CS_ERR_UNSUPPORTED, "fingerprints longer than 2^30 bits");
-    const FpLayout need = fp_layout(nullptr, n_refs, n_queries, n_words);
-    if (!workspace || workspace_bytes < need.bytes)
-        return fail(CS_ERR_WORKSPACE, "workspace of %zu bytes < %zu required", workspace_bytes,
-                    need.bytes);
-    const FpLayout L = fp_layout(workspace, n_refs, n_queries, n_words);
+    if (n_words * 32 > kMaxBits)
+        return fail(CS_ERR_UNSUPPORTED, "fingerprints longer than %lld bits",
+                    (long long)kMaxBits);
+    const int64_t gx = (n_queries + kQueryTile - 1) / kQueryTile;
+    const int64_t gy = (n_refs + kRefTile - 1) / kRefTile;
+    const int64_t n_tiles = gx * gy;
+    auto aligned16 = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; };
+    const bool full =
+        n_words % (2 * kStepWords) == 0 && aligned16(ref_words) && aligned16(query_words);
+    const int vec = ld % 4 == 0 && aligned16(out) && (!intersections || aligned16(intersections));
+    // Two resident workgroups per CU (256 threads, two waves per SIMD).
+    const int64_t grid = std::min<int64_t>(n_tiles, 2 * (int64_t)device_cu_count());
     hipStream_t hs = static_cast<hipStream_t>(stream);
-    auto blocks = [](int64_t n, int per) {
-        int64_t b = (n + per - 1) / per;
-        return (unsigned)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+    auto launch = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(kThreads), 0, hs, ref_words,
+                           n_refs, query_words, n_queries, n_words, gx, n_tiles, ref_valid,
+                           query_valid, empty_score, out, intersections, ld, vec);
     };
-    hipLaunchKernelGGL(k_fp_expand, dim3(blocks(L.rows_a * L.k_bytes / 4, 256)), dim3(256), 0, hs,
-                       ref_words, n_refs, L.rows_a, n_words, L.k_bytes,
-                       reinterpret_cast<uint32_t *>(L.A));
-    CS_LAUNCH_CHECK("k_fp_expand");
-    hipLaunchKernelGGL(k_fp_expand, dim3(blocks(L.rows_b * L.k_bytes / 4, 256)), dim3(256), 0, hs,
-                       query_words, n_queries, L.rows_b, n_words, L.k_bytes,
-                       reinterpret_cast<uint32_t *>(L.B));
-    CS_LAUNCH_CHECK("k_fp_expand");
-    hipLaunchKernelGGL(k_fp_popcount, dim3(blocks(n_refs, 256)), dim3(256), 0, hs, ref_words,
-                       n_refs, n_words, L.pa);
-    CS_LAUNCH_CHECK("k_fp_popcount");
-    hipLaunchKernelGGL(k_fp_popcount, dim3(blocks(n_queries, 256)), dim3(256), 0, hs,
-                       query_words, n_queries, n_words, L.pb);
-    CS_LAUNCH_CHECK("k_fp_popcount");
-    const int64_t gx = L.rows_b / kTile, gy = L.rows_a / kTile;
-    if (gx > INT_MAX || gy > 65535)
-        return fail(CS_ERR_UNSUPPORTED, "fingerprint grid too large");
-    hipLaunchKernelGGL(k_fp_gemm, dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), 0, hs, L.A,
-                       L.B, L.k_bytes, L.pa, L.pb, n_refs, n_queries, ref_valid, query_valid,
-                       empty_score, measure, out, intersections, ld);
+    if (full) {
+        if (measure == CS_JACCARD) launch(k_fp_gemm<true, CS_JACCARD>);
+        else if (measure == CS_DICE) launch(k_fp_gemm<true, CS_DICE>);
+        else launch(k_fp_gemm<true, CS_COSINE>);
+    } else {
+        if (measure == CS_JACCARD) launch(k_fp_gemm<false, CS_JACCARD>);
+        else if (measure == CS_DICE) launch(k_fp_gemm<false, CS_DICE>);
+        else launch(k_fp_gemm<false, CS_COSINE>);
+    }
     CS_LAUNCH_CHECK("k_fp_gemm");
     return CS_OK;
 }

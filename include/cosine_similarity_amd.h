@@ -222,15 +222,18 @@ typedef enum cs_fingerprint_measure {
     CS_COSINE = 2    /* I / sqrt(|a| * |b|)      vector_similarity_functions.py:116-118 */
 } cs_fingerprint_measure;
 
-/* Bytes of device workspace cs_fingerprint_similarity needs. */
+/* Bytes of device workspace cs_fingerprint_similarity needs (currently 0:
+ * the kernel reads the packed words directly; the workspace arguments are
+ * kept so callers need not change if a future kernel needs scratch). */
 cs_status cs_fingerprint_workspace_size(int64_t n_refs, int64_t n_queries, int64_t n_words,
                                         size_t *bytes);
 
 /*
  * out[r*ld + q] = measure(ref r, query q) as float32; 0/0 gives 0 (the
  * reference's nan_to_num).  Fingerprints are bit-packed, n_words uint32 words
- * per row (bit b of row x is word b/32, bit b%32).  Intersections and
- * popcounts are exact int32 (int8 matrix-core products of 0/1 tiles).  Rows
+ * per row (bit b of row x is word b/32, bit b%32); at most 2^23 bits
+ * (CS_ERR_UNSUPPORTED beyond).  Intersections and popcounts are exact (fp4
+ * matrix-core products of 0/1 bit planes, accumulated in float32).  Rows
  * whose valid flag is 0 (missing fingerprint) get `empty_score` instead
  * (CudaFingerprintSimilarity.py:147-154); a NULL valid array means all valid.
  * If `intersections` is not NULL it receives the int32 counts [r*ld + q].

@@ -67,24 +67,33 @@ def test_missing_fingerprint(gpu, case):
     assert np.allclose(m, want, equal_nan=True)
 
 
-@pytest.mark.parametrize("measure", ["jaccard", "dice", "cosine"])
-def test_random_vs_oracle(gpu, oracle, measure):
+@pytest.mark.parametrize("measure,n_bits,pad", [
+    ("jaccard", 1000, 3), ("dice", 1000, 3), ("cosine", 1000, 3),
+    # 10 words (one partial 512-bit step), 80 words (odd step count): the
+    # bounded-load path; pad 1 makes the row stride a multiple of 4 floats,
+    # so the 16-byte stores run with a ragged last group of 4 queries.
+    ("jaccard", 300, 1), ("jaccard", 2560, 1), ("dice", 2560, 3),
+])
+def test_random_vs_oracle(gpu, oracle, measure, n_bits, pad):
     """Ragged shapes (not multiples of the 128 tile, bits not a multiple of 64)."""
     import torch
 
     from cudams import synthetic
     from cudams.device import fingerprint_similarity
 
-    a = synthetic.fingerprints(300, 3, n_bits=1000, zero_rows=3)
-    b = synthetic.fingerprints(211, 4, n_bits=1000, zero_rows=2)
+    a = synthetic.fingerprints(300, 3, n_bits=n_bits, zero_rows=3)
+    b = synthetic.fingerprints(211, 4, n_bits=n_bits, zero_rows=2)
     wa = torch.from_numpy(synthetic.pack_bits(a).view(np.int32)).cuda()
     wb = torch.from_numpy(synthetic.pack_bits(b).view(np.int32)).cuda()
-    out = torch.empty((300, 211 + 3), dtype=torch.float32, device="cuda")
-    inter = torch.empty((300, 211 + 3), dtype=torch.int32, device="cuda")
+    out = torch.full((300, 211 + pad), -7.0, dtype=torch.float32, device="cuda")
+    inter = torch.full((300, 211 + pad), -7, dtype=torch.int32, device="cuda")
     fingerprint_similarity(measure, wa, wb, out, intersections=inter)
     torch.cuda.synchronize()
     want, wi = oracle.fingerprint_matrix(a, b, measure)
     np.testing.assert_array_equal(inter.cpu().numpy()[:, :211], wi)
+    # columns past the last query are left alone
+    assert (out.cpu().numpy()[:, 211:] == -7.0).all()
+    assert (inter.cpu().numpy()[:, 211:] == -7).all()
     got = out.cpu().numpy()[:, :211]
     if measure == "cosine":
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-6)
