@@ -515,6 +515,81 @@ struct RefChunk {
     double in0, in1;
 };
 
+// LDS layout of the lane kernel's per-wave candidate pool (see
+// light_lds_bytes): values, (i, j), per-query counts, pool count, compaction
+// slots, query of each slot.
+struct WavePool {
+    double *pv;
+    uint32_t *pij;
+    int *cnt;
+    int *pool_n;
+    int *midx;
+    unsigned char *pq;
+};
+
+__device__ __forceinline__ WavePool wave_pool(unsigned char *lds, int wave) {
+    int64_t *qoff_l = reinterpret_cast<int64_t *>(lds);
+    double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
+    int32_t *roff_l = reinterpret_cast<int32_t *>(qnorm_l + kTileQueries);
+    unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kMaxLightRefs + 1);
+    WavePool w;
+    w.pv = reinterpret_cast<double *>(pools + wave * kWaveLds);
+    w.pij = reinterpret_cast<uint32_t *>(w.pv + kPool);
+    w.cnt = reinterpret_cast<int *>(w.pij + kPool);
+    w.pool_n = w.cnt + kTileQueries;
+    w.midx = w.pool_n + 4;
+    w.pq = reinterpret_cast<unsigned char *>(w.midx + kWave);
+    return w;
+}
+
+struct MidResult {
+    double sc;  // greedy score sum (not yet normalised)
+    int m;      // matches
+};
+
+// One pair with K < n <= kMidCap candidates in the wave's pool (slots
+// 0 .. lim - 1, owner lane sl), settled by the whole wave: the pair's slots
+// are compacted, their (value, i, j) sorted descending across the lanes, and
+// the greedy pass accepts the first live candidate, then drops every one that
+// shares its i or j.  The same result on every lane.
+__device__ __attribute__((noinline)) MidResult mid_settle(int wave, int lim, int sl) {
+    extern __shared__ __align__(16) unsigned char light_lds[];
+    const WavePool P = wave_pool(light_lds, wave);
+    const int lane = threadIdx.x & (kWave - 1);
+    int have = 0;
+    for (int base = 0; base < lim; base += kWave) {  // compact the pair's entries
+        const int e = base + lane;
+        const bool mine_e = e < lim && P.pq[e] == sl;
+        const unsigned long long mk = __ballot(mine_e);
+        // at most kMidCap entries belong to sl, so every slot is < 64
+        if (mine_e) P.midx[have + __popcll(mk & ((1ull << lane) - 1ull))] = e;
+        have += __popcll(mk);
+    }
+    wave_sync();
+    double v = -__builtin_huge_val();
+    uint32_t ij = 0u;
+    if (lane < have) {
+        const int e = P.midx[lane];
+        v = P.pv[e];
+        ij = P.pij[e];
+    }
+    wave_sync();
+    wave_sort_desc(v, ij);
+    const int vi = (int)(ij >> 16), vj = (int)(ij & 0xffffu);
+    bool alive = lane < have;
+    MidResult r{0.0, 0};  // the same on every lane (read from lane f)
+    unsigned long long mk = __ballot(alive);
+    while (mk) {
+        const int f = __builtin_ctzll(mk);
+        const int fi = lane_int(vi, f), fj = lane_int(vj, f);
+        r.sc += lane_double(v, f);
+        ++r.m;
+        alive = alive & (lane != f) & (vi != fi) & (vj != fj);
+        mk = __ballot(alive);
+    }
+    return r;
+}
+
 // One workgroup = one 64-query tile x refs_per_block references.  The tile's peak
 // index sits in LDS (shared by 16 waves); wave w settles references taken
 // from a workgroup counter one at a time: lanes over the reference's peaks
@@ -822,46 +897,16 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                 out_overflow = n >= a.match_limit;
             }
             // 2b. K < n <= 64: the whole wave settles the pair from registers
+            // (out of line: its registers would otherwise be held, and
+            // spilled, across every step)
             unsigned long long mid = __ballot(settle && n > K);
             while (mid) {
                 const int sl = __builtin_ctzll(mid);
                 mid &= mid - 1ull;
-                int have = 0;
-                const int lim = used < kPool ? used : kPool;
-                for (int base = 0; base < lim; base += kWave) {  // compact the pair's entries
-                    const int e = base + lane;
-                    const bool mine_e = e < lim && pq[e] == sl;
-                    const unsigned long long mk = __ballot(mine_e);
-                    // at most kMidCap entries belong to sl, so every slot is < 64
-                    if (mine_e) midx[have + __popcll(mk & ((1ull << lane) - 1ull))] = e;
-                    have += __popcll(mk);
-                }
-                wave_sync();
-                double v = -__builtin_huge_val();
-                uint32_t ij = 0u;
-                if (lane < have) {
-                    const int e = midx[lane];
-                    v = pv[e];
-                    ij = pij[e];
-                }
-                wave_sync();
-                wave_sort_desc(v, ij);
-                const int vi = (int)(ij >> 16), vj = (int)(ij & 0xffffu);
-                bool alive = lane < have;
-                double sc = 0.0;  // the same on every lane (read from lane f)
-                int m = 0;
-                unsigned long long mk = __ballot(alive);
-                while (mk) {
-                    const int f = __builtin_ctzll(mk);
-                    const int fi = lane_int(vi, f), fj = lane_int(vj, f);
-                    sc += lane_double(v, f);
-                    ++m;
-                    alive = alive & (lane != f) & (vi != fi) & (vj != fj);
-                    mk = __ballot(alive);
-                }
+                const MidResult res = mid_settle(wave, used < kPool ? used : kPool, sl);
                 if (lane == sl) {
-                    out_score = (float)(sc / (rnorm * qnorm_l[sl]));
-                    out_matches = m;
+                    out_score = (float)(res.sc / (rnorm * qnorm_l[sl]));
+                    out_matches = res.m;
                     out_overflow = n >= a.match_limit;
                 }
             }
