@@ -479,25 +479,62 @@ constexpr int kLightCap = (int)((kLdsBytes - light_lds_bytes(0) - 64) / kLightPe
 static_assert(light_lds_bytes(kLightCap) <= kLdsBytes, "lane kernel LDS fits a workgroup");
 
 // Descending bitonic sort of one key per lane across the wave (64 lanes).
+// x from lane (lane ^ J).  J <= 8 stays within a 16-lane row and uses DPP
+// lane moves (VALU, no LDS round trip): quad_perm [1,0,3,2] and [2,3,0,1] for
+// 1 and 2, row_half_mirror then quad_perm [3,2,1,0] for 4 (lane i of a
+// half-row reads 7 - i, then quad lane k reads 3 - k: i ^ 4), row_ror:8 for
+// 8.  J = 16 is a ds_swizzle xor within 32 lanes, J = 32 a bpermute.
+template <int J>
+__device__ __forceinline__ int xor_lane(int x) {
+    if constexpr (J == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
+    if constexpr (J == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
+    if constexpr (J == 4)
+        return __builtin_amdgcn_update_dpp(
+            0, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
+    if constexpr (J == 8) return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);
+    if constexpr (J == 16) return __builtin_amdgcn_ds_swizzle(x, 0x401F);
+    return __shfl_xor(x, 32);
+}
+
+template <int J>
+__device__ __forceinline__ double xor_lane_double(double x) {
+    const int lo = xor_lane<J>(__double2loint(x));
+    const int hi = xor_lane<J>(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
+// One compare-exchange stage of the bitonic network (block size Kb, partner
+// distance J).
+template <int Kb, int J>
+__device__ __forceinline__ void sort_stage(double &v, uint32_t &ij, int lane) {
+    const double ov = xor_lane_double<J>(v);
+    const uint32_t oij = (uint32_t)xor_lane<J>((int)ij);
+    const bool lower = (lane & J) == 0;
+    const bool desc = (lane & Kb) == 0;
+    // the lower index of a descending block keeps the greater key
+    const bool keep_greater = lower == desc;
+    const bool other_greater = key_greater(ov, oij, v, ij);
+    const bool mine_greater = key_greater(v, ij, ov, oij);
+    const bool take = keep_greater ? other_greater : mine_greater;
+    v = take ? ov : v;
+    ij = take ? oij : ij;
+}
+
+template <int Kb, int J>
+__device__ __forceinline__ void sort_merge(double &v, uint32_t &ij, int lane) {
+    sort_stage<Kb, J>(v, ij, lane);
+    if constexpr (J > 1) sort_merge<Kb, J / 2>(v, ij, lane);
+}
+
+// Bitonic sort of one (value, ij) per lane across the wave, descending by key.
 __device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
     const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int k = 2; k <= kWave; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const double ov = __shfl_xor(v, j);
-            const uint32_t oij = __shfl_xor(ij, j);
-            const bool lower = (lane & j) == 0;
-            const bool desc = (lane & k) == 0;
-            // the lower index of a descending block keeps the greater key
-            const bool keep_greater = lower == desc;
-            const bool other_greater = key_greater(ov, oij, v, ij);
-            const bool mine_greater = key_greater(v, ij, ov, oij);
-            const bool take = keep_greater ? other_greater : mine_greater;
-            v = take ? ov : v;
-            ij = take ? oij : ij;
-        }
-    }
+    sort_merge<2, 1>(v, ij, lane);
+    sort_merge<4, 2>(v, ij, lane);
+    sort_merge<8, 4>(v, ij, lane);
+    sort_merge<16, 8>(v, ij, lane);
+    sort_merge<32, 16>(v, ij, lane);
+    sort_merge<64, 32>(v, ij, lane);
 }
 
 // s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited on (gfx9 layout:
