@@ -111,31 +111,43 @@ __device__ __forceinline__ const PrepJob &prep_job(const PrepJob &one, const Pre
     return j;
 }
 
-// Per tile: peak count, position of its sorted list, value range.
-__global__ void k_tile_meta(PrepJob one, const PrepJob *jobs, const int32_t *tile_job,
-                            int64_t ntiles, int32_t *tile_np, int64_t *tile_pbase,
-                            float *tile_lo, float *tile_iw) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < ntiles;
-         g += (int64_t)gridDim.x * blockDim.x) {
+// Per tile: peak count, position of its sorted list, value range.  One wave
+// per tile, lane l reading query l's first and last peak; the range is a
+// wave min / max (the per-query lists are sorted ascending).
+__global__ void __launch_bounds__(256)
+k_tile_meta(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
+            int32_t *tile_np, int64_t *tile_pbase, float *tile_lo, float *tile_iw) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t g = (int64_t)blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave); g < ntiles;
+         g += waves) {
         int64_t t;
         const PrepJob &J = prep_job(one, jobs, tile_job, g, t);
         const float *mz = J.mz;
         const int64_t *off = J.off;
         const int64_t q0 = t * kTileQueries;
         const int64_t q1 = q0 + kTileQueries < J.nq ? q0 + kTileQueries : J.nq;
+        const int64_t q = q0 + lane;
         float lo = __builtin_huge_valf(), hi = -__builtin_huge_valf();
-        for (int64_t q = q0; q < q1; ++q) {
-            if (off[q + 1] > off[q]) {
-                const float a = mz[off[q]] + J.shift32, b = mz[off[q + 1] - 1] + J.shift32;
-                lo = a < lo ? a : lo;
-                hi = b > hi ? b : hi;
+        if (q < q1) {
+            const int64_t a = off[q], b = off[q + 1];
+            if (b > a) {
+                lo = mz[a] + J.shift32;
+                hi = mz[b - 1] + J.shift32;
             }
         }
-        if (!(lo <= hi)) lo = hi = 0.0f;
-        tile_np[g] = (int32_t)(off[q1] - off[q0]);
-        tile_pbase[g] = J.pos_base + (off[q0] - off[0]) + t;
-        tile_lo[g] = lo;
-        tile_iw[g] = tile_inv_width(lo, hi);
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const float ol = __shfl_xor(lo, o), oh = __shfl_xor(hi, o);
+            lo = ol < lo ? ol : lo;
+            hi = oh > hi ? oh : hi;
+        }
+        if (lane == 0) {
+            if (!(lo <= hi)) lo = hi = 0.0f;
+            tile_np[g] = (int32_t)(off[q1] - off[q0]);
+            tile_pbase[g] = J.pos_base + (off[q0] - off[0]) + t;
+            tile_lo[g] = lo;
+            tile_iw[g] = tile_inv_width(lo, hi);
+        }
     }
 }
 
@@ -260,15 +272,26 @@ k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t 
             loc[u] = start[tid * (kTileBuckets / kIndexThreads) + u];
             sum += loc[u];
         }
-        part[tid] = sum;
-        __syncthreads();
-        for (int o = 1; o < kIndexThreads; o <<= 1) {
-            const int v = tid >= o ? part[tid - o] : 0;
-            __syncthreads();
-            part[tid] += v;
-            __syncthreads();
+        // inclusive scan of the thread sums: within each wave by shuffles,
+        // then over the 16 wave totals
+        const int lane = tid & (kWave - 1), wv = tid / kWave;
+        int incl = sum;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(incl, o);
+            if (lane >= o) incl += y;
         }
-        int run = tid ? part[tid - 1] : 0;
+        if (lane == kWave - 1) part[wv] = incl;
+        __syncthreads();
+        if (tid < kWave) {
+            int w = tid < kIndexThreads / kWave ? part[tid] : 0;
+            for (int o = 1; o < kIndexThreads / kWave; o <<= 1) {
+                const int y = __shfl_up(w, o);
+                if (tid >= o) w += y;
+            }
+            if (tid < kIndexThreads / kWave) part[kIndexThreads / kWave + tid] = w;
+        }
+        __syncthreads();
+        int run = incl - sum + (wv ? part[kIndexThreads / kWave + wv - 1] : 0);
 #pragma unroll
         for (int u = 0; u < kTileBuckets / kIndexThreads; ++u) {
             const int b = tid * (kTileBuckets / kIndexThreads) + u;
@@ -282,8 +305,9 @@ k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t 
         for (int b = tid; b <= kTileBuckets; b += kIndexThreads) d[b] = start[b];
         for (int e = tid; e < np; e += kIndexThreads) {  // scatter into buckets
             const int64_t gp = g0 + e;
-            int ql = 0;
-            while (ql + 1 < nl && loff[ql + 1] <= gp) ++ql;
+            int ql = 0;  // last list whose offset is <= gp (binary search)
+            for (int step = kTileQueries / 2; step > 0; step >>= 1)
+                if (ql + step < nl && loff[ql + step] <= gp) ql += step;
             const float x = mz[gp] + shift32;
             const int pos = atomicAdd(&cursor[tile_bucket(x, lo, iw)], 1);
             bx[pos] = x;
@@ -330,7 +354,7 @@ k_tile_dir(int64_t ntiles, const int32_t *tile_np, const int64_t *tile_pbase,
 // sint[p] = float32 of the int_pow of list position p, and whether every
 // value of the tile survives that rounding (then the lane kernel forms
 // candidate values from these copies exactly).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_tile_values(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
               const int32_t *tile_np, const int64_t *tile_pbase, const uint32_t *swho,
               float *sint, int32_t *tile_flags) {
@@ -1702,7 +1726,7 @@ cs_status launch_prepare(const PrepJob &one, const PrepJob *jobs, const int32_t 
         int64_t b = (n + per - 1) / per;
         return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
     };
-    hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256)), dim3(256), 0, hs, one, jobs,
+    hipLaunchKernelGGL(k_tile_meta, dim3(blocks(L.ntiles, 256 / kWave)), dim3(256), 0, hs, one, jobs,
                        tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw);
     CS_LAUNCH_CHECK("k_tile_meta");
     const size_t ilds = sizeof(int) * (2 * kTileBuckets + 1) + 8 * (size_t)kIndexStage;
@@ -1723,7 +1747,7 @@ cs_status launch_prepare(const PrepJob &one, const PrepJob *jobs, const int32_t 
     hipLaunchKernelGGL(k_tile_dir, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, L.ntiles,
                        L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz, L.dir);
     CS_LAUNCH_CHECK("k_tile_dir");
-    hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, one, jobs,
+    hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1)), dim3(1024), 0, hs, one, jobs,
                        tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.swho, L.sint,
                        L.tile_flags);
     CS_LAUNCH_CHECK("k_tile_values");
