@@ -28,6 +28,7 @@ sys.path.insert(0, os.path.join(REPO, "cosine-similarity_amd"))
 
 HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
 FP4_PEAK_TFLOPS = 10100.0  # MI355X dense fp4 MFMA
+INT8_PEAK_TOPS = 5000.0    # MI355X dense int8 MFMA
 
 
 def log(msg: str) -> None:
@@ -76,9 +77,46 @@ def cpu_baseline_cosine(packed, tol, seconds_target: float):
         rows = want
         t = stripe(rows)
         rate = rows * n / t
+    # one core: the per-core rate, on a stripe sized to about 2 s
+    rows1 = max(1, min(n, int(rate / cores * 2.0 / n)))
+    t1 = time.perf_counter()
+    sub1 = (packed.mz[: packed.offsets[rows1]], packed.intensities[: packed.offsets[rows1]],
+            packed.offsets[: rows1 + 1])
+    oracle.cosine_greedy_matrix(sub1, csr, tolerance=tol, nthreads=1)
+    t1 = time.perf_counter() - t1
     return {"value": rate, "unit": "pairs/s", "cores": cores, "kind": "port",
             "sample": f"first {rows} reference rows x all {n} queries ({rows * n} pairs), "
-                      f"{t:.1f} s, oracle/cs_oracle.c float64, OpenMP"}
+                      f"{t:.1f} s, oracle/cs_oracle.c float64, OpenMP",
+            "single_core": {"value": rows1 * n / t1, "unit": "pairs/s", "cores": 1,
+                            "sample": f"first {rows1} reference rows x all {n} queries, "
+                                      f"{t1:.1f} s"}}
+
+
+def cpu_baseline_fingerprint(fa, fb, seconds_target: float):
+    """The oracle's float64 fingerprint restatement (jaccard, OpenMP over rows)
+    on a stripe of reference rows against all queries."""
+    from oracle import oracle
+
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    n = fa.shape[0]
+
+    def stripe(rows):
+        t0 = time.perf_counter()
+        oracle.fingerprint_matrix(fa[:rows], fb, "jaccard", nthreads=cores)
+        return time.perf_counter() - t0
+
+    rows = 4
+    t = stripe(rows)
+    while t < 0.3 and rows < n:
+        rows = min(n, rows * 4)
+        t = stripe(rows)
+    want = int(min(n, max(rows, rows * seconds_target / t)))
+    if want > rows:
+        rows = want
+        t = stripe(rows)
+    return {"value": rows * fb.shape[0] / t, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"first {rows} reference rows x all {fb.shape[0]} queries, {t:.1f} s, "
+                      f"oracle/cs_oracle.c fingerprint_matrix (jaccard), OpenMP"}
 
 
 def bench_cosine(a) -> dict:
@@ -199,8 +237,12 @@ def bench_fingerprint(a) -> dict:
         "config": {"workload": "fingerprint_4096x4096x2048_jaccard"},
         "roofline": {"bound": "mfma", "achieved": ops / (ms / 1e3) / 1e12, "peak": FP4_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": ops / (ms / 1e3) / 1e12 / FP4_PEAK_TFLOPS,
-                     "traffic": None},
-        "cpu_baseline": None,
+                     "traffic": None,
+                     # the same call against the int8 MFMA peak, and the
+                     # float32 output write against HBM
+                     "int8_frac": ops / (ms / 1e3) / 1e12 / INT8_PEAK_TOPS,
+                     "output_hbm_frac": 4.0 * n * n / (ms / 1e3) / 8.0e12},
+        "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline_fingerprint(fa, fb, 5.0),
     }
     return res
 
