@@ -846,12 +846,27 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     if (rank < kIdxRanks) pidx[rank * kTileQueries + ql] = (unsigned char)pos;
                 };
                 if (in_lds) {
-                    tile_probe(x0, x1, lo, iw, [&](int p) { return smz_l[p]; },
-                               [&](int b) { return (int)dir_l[b]; },
-                               [&](int p) {
-                                   emit((int)wl_l[p], (int)wj_l[p], smz_l[p],
-                                        TF ? sint_l[p] : 0.0f);
-                               });
+                    // tile_probe on the LDS copy, with the skip done as a
+                    // binary search inside the bucket: positions before
+                    // dir[b] hold values below x0 and positions from dir[b+1]
+                    // on hold values above it (bucket() is monotone), so the
+                    // first value >= x0 sits in [dir[b], dir[b+1]].  About 3
+                    // rounds per wave instead of about 8 for the linear walk.
+                    const int b = tile_bucket(x0, lo, iw);
+                    int pl = (int)dir_l[b], ph = (int)dir_l[b + 1];
+                    while (pl < ph) {
+                        const int mid = (pl + ph) >> 1;
+                        if (smz_l[mid] < x0)
+                            pl = mid + 1;
+                        else
+                            ph = mid;
+                    }
+                    int p = pl;
+                    float v = smz_l[p];
+                    while (v <= x1) {
+                        emit((int)wl_l[p], (int)wj_l[p], v, TF ? sint_l[p] : 0.0f);
+                        v = smz_l[++p];
+                    }
                 } else {
                     tile_probe(x0, x1, lo, iw, [&](int p) { return gsmz[p]; },
                                [&](int b) { return (int)gdir[b]; },
