@@ -943,15 +943,16 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             const bool settle = mine && !dropped && n <= kMidCap;
             // 2a. up to K candidates: walk the pair's list into registers
             const bool small = settle && n <= K;
-            // wave-wide max of the small counts, by ballots (no LDS round trip);
-            // a scalar, so the unrolled steps past it are branched over
-            int bound = 0;
-#pragma unroll
-            for (int b = K; b > 0; --b) {
-                if (__ballot(small && n >= b)) {
-                    bound = b;
-                    break;
-                }
+            // wave-wide max of the small counts, by a binary search of ballots
+            // (no LDS round trip); a scalar, so the unrolled steps past it are
+            // branched over
+            int bound = 0, top = K;
+            while (bound < top) {
+                const int mid = (bound + top + 1) >> 1;
+                if (__ballot(small && n >= mid))
+                    bound = mid;
+                else
+                    top = mid - 1;
             }
             static_assert(K <= kIdxRanks, "every small pair's candidates are indexed by rank");
             LaneCandidates<K> cand;
