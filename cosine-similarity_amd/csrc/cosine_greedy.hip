@@ -1109,8 +1109,13 @@ __device__ void sort_desc(double *bv, uint32_t *bij, int n) {
     }
     __syncthreads();
     for (int size = 2; size <= n2; size <<= 1) {
+        // blocks of `size` entries that lie wholly in the -inf padding stay
+        // padding through this size's stages: their compare-exchanges are
+        // skipped (only blocks up to the one holding entry n - 1 are sorted)
+        const int live = ((n + size - 1) / size) * size;
+        const int pairs = (live < n2 ? live : n2) >> 1;
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = tid; t < (n2 >> 1); t += nt) {
+            for (int t = tid; t < pairs; t += nt) {
                 const int lo = 2 * t - (t & (stride - 1));
                 const int hi = lo + stride;
                 const bool desc = (lo & size) == 0;
