@@ -324,6 +324,31 @@ def test_stage_reuse_with_other_references(gpu, oracle):
         assert_parity(res, oracle_csr(oracle, refs, lib, tolerance=0.01))
 
 
+def test_heavy_sort_widths(gpu, oracle):
+    """Pairs whose candidate counts fall in every padded sort size of the
+    heavy kernel (512, 1024, 2048, 4096 entries: register passes of 2, 4, 8
+    and 16 entries per thread) and below: dense peak packs inside a wide
+    tolerance give n_r x n_q candidates, with tied intensities."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(91)
+    mzs, its = [], []
+    for k in (12, 18, 22, 30, 34, 42, 48, 55, 60):
+        pack = 300.0 + rng.uniform(0, 0.8, k).astype(np.float32)
+        rest = rng.uniform(400.0, 900.0, 20).astype(np.float32)
+        mzs.append(np.sort(np.concatenate([pack, rest])).astype(np.float64))
+        its.append(np.round(rng.random(k + 20) * 8 + 1) / 8)  # ties
+    p = PackedSpectra.from_arrays(mzs, its)
+    say("heavy sort widths")
+    res = device_run(p, p, tol=1.0, match_limit=5000)
+    want = oracle_csr(oracle, p, p, tolerance=1.0, match_limit=5000)
+    c = want["n_candidates"]
+    for lo, hi in ((65, 256), (257, 512), (513, 1024), (1025, 2048), (2049, 4096)):
+        assert ((c >= lo) & (c <= hi)).any(), (lo, hi)
+    assert c.max() <= 4096
+    assert_parity(res, want, 5000)
+
+
 @pytest.mark.parametrize("T", [4, 64, 1024])
 def test_giant_pairs(gpu, oracle, T):
     """Pairs with more candidates than the single-pass buffer (4096): dense
