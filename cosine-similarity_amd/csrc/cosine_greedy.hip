@@ -4,23 +4,23 @@
 // functions.py:39-262 and the per-tile host loop of CudaCosineGreedy.py:149-185.
 //
 // Pipeline (all asynchronous on the caller's stream, no host sync):
-//   1. k_fill            zero counters / histogram (a kernel, not a memset node,
-//                        so the sequence can sit in a captured graph)
-//   2. k_len_hist, k_scan_block, k_len_scatter
-//                        counting sort of the queries by peak count
-//   3. k_tile_len, k_scan_block, k_tile_build
-//                        groups of 64 length-sorted queries ("query tiles"),
-//                        stored interleaved [peak][lane] with a +inf pad row,
-//                        so lane l reads its own query with no LDS bank
-//                        conflicts whatever peak each lane is at
-//   4. k_light<K>        one lane per (reference, query) pair: float32 merge
-//                        scan with an exact float64 test on coarse hits; up to
-//                        K candidates are sorted and settled in registers
-//   5. k_heavy           one wave per pair that had more than K candidates:
-//                        parallel window enumeration, LDS bitonic sort by the
-//                        (v desc, i desc, j desc) key, wave-parallel greedy,
-//                        in rounds of at most T candidates, so any candidate
-//                        count is handled exactly in bounded LDS.
+//   1. k_fill_u64        zero the counters (a kernel, not a memset node, so the
+//                        sequence can sit in a captured graph)
+//   2. k_tile_meta, k_tile_index, k_tile_sort, k_tile_dir, k_tile_values
+//                        per 64-query tile: every peak's m/z in one sorted
+//                        list with its (query lane, peak) and a bucket
+//                        directory over the tile's m/z range
+//   3. k_light<K, F32>   lane l of a wave = pair (reference, query l of the
+//                        tile): each reference peak finds its window in the
+//                        tile list (directory + binary search), exact
+//                        candidates go to the wave's LDS pool; pairs with at
+//                        most K candidates settle in registers, up to 64 by
+//                        the whole wave, more are queued
+//   4. k_heavy           one workgroup per queued pair: enumeration into LDS,
+//                        block bitonic sort by the (v desc, i desc, j desc)
+//                        key, one-wave greedy
+//   5. k_giant           one wave per pair above the heavy buffer, in rounds
+//                        of at most T candidates (exact in bounded LDS)
 #include <limits.h>
 
 #include <algorithm>
@@ -480,7 +480,7 @@ constexpr int kIdxRanks = 16;   // >= the largest light_candidates
 constexpr int kMidCap = 64;     // candidates a wave settles for one pair in registers
 constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first
 constexpr size_t kWaveLds =
-    ((size_t)kPool * (8 + 4 + 1) + (size_t)kTileQueries * (4 + 1 + kIdxRanks) + 16 + kWave * 4 +
+    ((size_t)kPool * (8 + 4 + 1) + (size_t)kTileQueries * (4 + 1 + kIdxRanks + 1) + 16 + kWave * 4 +
      15) / 16 * 16;
 constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
 
@@ -684,7 +684,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     int *midx = pool_n + 4;
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
     unsigned char *pdrop = pq + kPool;  // query lost a candidate to a full pool
-    unsigned char *pidx = pdrop + kTileQueries;  // [kIdxRanks][kTileQueries] pool slots
+    unsigned char *pidx = pdrop + kTileQueries;  // [kIdxRanks + 1][kTileQueries] pool slots
     float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * kWaveLds);
     float *sint_l = smz_l + cap + 1;
     uint16_t *dir_l = reinterpret_cast<uint16_t *>(sint_l + cap + 1);
@@ -812,27 +812,31 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             auto chunk = [&](int ib, float ri, double rin) {
                 const int i = ib + lane;
                 if (i >= nr) return;
-                const float x0 = ri - a.tolc, x1 = ri + a.tolc;
+                // The walk covers [x0, x1].  With float32 m/z and no shift
+                // (F32) these are the float64 window itself as float32 bounds,
+                // so every entry walked is a candidate; otherwise a coarse
+                // float32 window that each entry's float64 test narrows.
+                float x0, x1;
+                if constexpr (F32)
+                    f32_window(ri, a.tol, x0, x1);
+                else
+                    x0 = ri - a.tolc, x1 = ri + a.tolc;
                 auto emit = [&](int ql, int j, float qv, float qin) {
                     if (!((filter >> ql) & 1ull)) return;
-                    double mq, mr;
-                    if constexpr (F32) {
-                        mq = (double)qv;
-                        mr = (double)ri;
-                    } else {
-                        mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qoff_l[ql] + j);
-                        mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+                    if constexpr (!F32) {
+                        const double mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qoff_l[ql] + j);
+                        const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+                        if (!exact_window(mr, mq, a.tol, a.shift)) return;
                     }
-                    if (!exact_window(mr, mq, a.tol, a.shift)) return;
                     // rank within the query and pool slot, both atomics in flight at once
                     const int rank = atomicAdd(&cnt[ql], 1);
                     const int pos = atomicAdd(pool_n, 1);
-                    if (rank >= kMidCap) {  // heavy anyway: the slot holds no candidate
-                        if (pos < kPool) pq[pos] = 0xffu;
-                        return;
-                    }
-                    if (pos >= kPool) {
-                        pdrop[ql] = 1;
+                    if ((rank >= kMidCap) | (pos >= kPool)) {  // rare: out of line
+                        if (rank >= kMidCap) {  // heavy anyway: the slot holds no candidate
+                            if (pos < kPool) pq[pos] = 0xffu;
+                        } else {
+                            pdrop[ql] = 1;
+                        }
                         return;
                     }
                     if constexpr (F32 && TF)  // ((1 * int_pow_r) * 1) * int_pow_q
@@ -843,7 +847,9 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                         pv[pos] = pair_value(R, roff + i, Q, qoff_l[ql] + j);
                     pij[pos] = pack_ij(i, j);
                     pq[pos] = (unsigned char)ql;
-                    if (rank < kIdxRanks) pidx[rank * kTileQueries + ql] = (unsigned char)pos;
+                    // ranks past the index go to a spare row nobody reads
+                    pidx[(rank < kIdxRanks ? rank : kIdxRanks) * kTileQueries + ql] =
+                        (unsigned char)pos;
                 };
                 if (in_lds) {
                     // tile_probe on the LDS copy, with the skip done as a
