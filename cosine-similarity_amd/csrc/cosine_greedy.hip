@@ -38,7 +38,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kLightWaves = 16;
 constexpr int kLightThreads = kWave * kLightWaves;
-constexpr int kMaxLightRefs = 1024;   // largest refs_per_block
+constexpr int kMaxLightRefs = 2048;   // largest refs_per_block
 constexpr int kDefaultK = 8;
 constexpr int kDefaultRound = 256;
 constexpr int kHeavyGrid = 4096;

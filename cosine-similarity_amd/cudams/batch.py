@@ -170,7 +170,7 @@ def signature(jobs: Sequence[CosineJob]) -> str:
 
 DEFAULT_SPACE = {
     "light_candidates": (4, 8),
-    "refs_per_block": (64, 128, 256, 512, 1024),
+    "refs_per_block": (64, 128, 256, 512, 1024, 2048),
     "round_capacity": (256,),
 }
 

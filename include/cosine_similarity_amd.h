@@ -88,7 +88,7 @@ typedef struct cs_cosine_params {
     int32_t round_capacity;  /* candidates the cooperative path settles per round
                                 (power of two, 4..1024); 0 = library default      */
     int32_t refs_per_block;  /* references one lane-kernel workgroup walks against
-                                its query tile, 1..1024; 0 = library default      */
+                                its query tile, 1..2048; 0 = library default      */
     int32_t stages;          /* 0 = whole pipeline, else a mask of CS_STAGE_*:
                                 calls with PREPARE, then LIGHT, then HEAVY on one
                                 stream and one workspace equal one call with 0.
@@ -175,7 +175,7 @@ typedef struct cs_cosine_job {
 typedef struct cs_batch_params {
     int32_t light_candidates;  /* as cs_cosine_params.light_candidates          */
     int32_t round_capacity;    /* as cs_cosine_params.round_capacity            */
-    int32_t refs_per_block;    /* references of one work item (x 64 queries), 1..1024;
+    int32_t refs_per_block;    /* references of one work item (x 64 queries), 1..2048;
                                   default 256                                     */
     int32_t grid;              /* persistent workgroups of the lane kernel;
                                   default one per compute unit                    */
