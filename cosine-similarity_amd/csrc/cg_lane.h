@@ -48,46 +48,6 @@ CS_HD bool exact_window(double mz_r, double mz_q, double tol, double shift) {
     return !(x > mz_r + tol) & !(x < mz_r - tol);
 }
 
-// Neighbours of a float32 value: toward -inf and toward +inf (f not NaN; the
-// step from -0 or +0 goes to the smallest subnormal of the right sign).
-CS_HD float f32_toward_neg(float f) {
-    uint32_t u;
-    __builtin_memcpy(&u, &f, 4);
-    u = (u & 0x7fffffffu) == 0u ? 0x80000001u : (u >> 31) ? u + 1u : u - 1u;
-    __builtin_memcpy(&f, &u, 4);
-    return f;
-}
-CS_HD float f32_toward_pos(float f) {
-    uint32_t u;
-    __builtin_memcpy(&u, &f, 4);
-    u = (u & 0x7fffffffu) == 0u ? 0x00000001u : (u >> 31) ? u - 1u : u + 1u;
-    __builtin_memcpy(&f, &u, 4);
-    return f;
-}
-
-// Largest float32 <= x and smallest float32 >= x, for finite x.  A float32
-// value y satisfies y <= x iff y <= f32_down(x), and y >= x iff y >= f32_up(x).
-CS_HD float f32_down(double x) {
-    const float f = (float)x;
-    return (double)f > x ? f32_toward_neg(f) : f;
-}
-CS_HD float f32_up(double x) {
-    const float f = (float)x;
-    return (double)f < x ? f32_toward_pos(f) : f;
-}
-
-// The float64 window of exact_window(mz_r, mz_q, tol, 0.0) for a float32
-// reference m/z, as float32 bounds: for every float32 mz_q,
-//   exact_window((double)mz_r, (double)mz_q, tol, 0.0)
-//     == (mz_q >= lo) & (mz_q <= hi)
-// with lo = f32_up(mz_r - tol) and hi = f32_down(mz_r + tol), both sums rounded
-// in float64 exactly as exact_window rounds them (mz_q + 0.0 == mz_q).
-CS_HD void f32_window(float mz_r, double tol, float &lo, float &hi) {
-    const double m = (double)mz_r;
-    lo = f32_up(m - tol);
-    hi = f32_down(m + tol);
-}
-
 // Up to K candidates kept sorted by key_greater (descending) in registers.
 // All indexing is compile-time after unrolling, so the arrays stay in VGPRs.
 template <int K>

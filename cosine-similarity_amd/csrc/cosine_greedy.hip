@@ -812,22 +812,20 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             auto chunk = [&](int ib, float ri, double rin) {
                 const int i = ib + lane;
                 if (i >= nr) return;
-                // The walk covers [x0, x1].  With float32 m/z and no shift
-                // (F32) these are the float64 window itself as float32 bounds,
-                // so every entry walked is a candidate; otherwise a coarse
-                // float32 window that each entry's float64 test narrows.
-                float x0, x1;
-                if constexpr (F32)
-                    f32_window(ri, a.tol, x0, x1);
-                else
-                    x0 = ri - a.tolc, x1 = ri + a.tolc;
+                // the walk covers the coarse float32 window [x0, x1]; the
+                // float64 test of each entry narrows it to the exact one
+                const float x0 = ri - a.tolc, x1 = ri + a.tolc;
                 auto emit = [&](int ql, int j, float qv, float qin) {
                     if (!((filter >> ql) & 1ull)) return;
-                    if constexpr (!F32) {
-                        const double mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qoff_l[ql] + j);
-                        const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
-                        if (!exact_window(mr, mq, a.tol, a.shift)) return;
+                    double mq, mr;
+                    if constexpr (F32) {
+                        mq = (double)qv;
+                        mr = (double)ri;
+                    } else {
+                        mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qoff_l[ql] + j);
+                        mr = R.mzx ? R.mzx[roff + i] : (double)ri;
                     }
+                    if (!exact_window(mr, mq, a.tol, a.shift)) return;
                     // rank within the query and pool slot, both atomics in flight at once
                     const int rank = atomicAdd(&cnt[ql], 1);
                     const int pos = atomicAdd(pool_n, 1);

@@ -49,8 +49,7 @@ def tile_harness():
 
     L = ctypes.CDLL(build_tile_harness())
     d, p, i = ctypes.c_double, ctypes.c_void_p, ctypes.c_int
-    L.tile_scan.argtypes = [i, p, p, p, i, p, p, p, i, d, d, d, d, d, i, p, p, p, p]
-    L.f32_window_mismatches.argtypes = [p, p, p, i]
+    L.tile_scan.argtypes = [i, p, p, p, i, p, p, p, i, d, d, d, d, d, p, p, p, p]
     L.coarse_tolerance.argtypes = [d, d, d]
     L.coarse_tolerance.restype = ctypes.c_float
     return L

@@ -23,8 +23,8 @@ struct Entry {
 template <int K>
 void run(const double *rmz, const double *rint, const int64_t *roff, int R, const double *qmz,
          const double *qint, const int64_t *qoff, int Q, double tol, double shift, double mzp,
-         double inp, double max_abs_mz, int f32win, double *score, int *matches,
-         int *candidates, int *heavy) {
+         double inp, double max_abs_mz, double *score, int *matches, int *candidates,
+         int *heavy) {
     const float tolc = cs::coarse_tolerance(tol, shift, max_abs_mz);
     const float shift32 = (float)shift;
     auto wmz = [&](double m) { return mzp == 0.0 ? 1.0 : pow(m, mzp); };
@@ -68,16 +68,11 @@ void run(const double *rmz, const double *rint, const int64_t *roff, int R, cons
             for (int64_t g = roff[r]; g < roff[r + 1]; ++g) {
                 const int i = (int)(g - roff[r]);
                 const float ri = (float)rmz[g];
-                // f32win: the F32 lane kernel's window (float32 m/z, no shift),
-                // walked with no float64 test
-                float x0 = ri - tolc, x1 = ri + tolc;
-                if (f32win) cs::f32_window(ri, tol, x0, x1);
-                cs::tile_probe(x0, x1, lo, iw, [&](int p) { return smz[p]; },
+                cs::tile_probe(ri - tolc, ri + tolc, lo, iw, [&](int p) { return smz[p]; },
                                [&](int b) { return dir[b]; }, [&](int p) {
                                    const int ql = (int)(who[p] >> 16), j = (int)(who[p] & 0xffffu);
                                    const int64_t qp = qoff[q0 + ql] + j;
-                                   if (!f32win && !cs::exact_window(rmz[g], qmz[qp], tol, shift))
-                                       return;
+                                   if (!cs::exact_window(rmz[g], qmz[qp], tol, shift)) return;
                                    double v = wmz(rmz[g]) * pow(rint[g], inp);
                                    v = v * wmz(qmz[qp]);
                                    v = v * pow(qint[qp], inp);
@@ -137,30 +132,16 @@ void run(const double *rmz, const double *rint, const int64_t *roff, int R, cons
 extern "C" int tile_scan(int K, const double *rmz, const double *rint, const int64_t *roff, int R,
                          const double *qmz, const double *qint, const int64_t *qoff, int Q,
                          double tol, double shift, double mzp, double inp, double max_abs_mz,
-                         int f32win, double *score, int *matches, int *candidates, int *heavy) {
+                         double *score, int *matches, int *candidates, int *heavy) {
     switch (K) {
-        case 1: run<1>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, f32win, score, matches, candidates, heavy); return 0;
-        case 4: run<4>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, f32win, score, matches, candidates, heavy); return 0;
-        case 8: run<8>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, f32win, score, matches, candidates, heavy); return 0;
-        case 16: run<16>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, f32win, score, matches, candidates, heavy); return 0;
+        case 1: run<1>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, score, matches, candidates, heavy); return 0;
+        case 4: run<4>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, score, matches, candidates, heavy); return 0;
+        case 8: run<8>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, score, matches, candidates, heavy); return 0;
+        case 16: run<16>(rmz, rint, roff, R, qmz, qint, qoff, Q, tol, shift, mzp, inp, max_abs_mz, score, matches, candidates, heavy); return 0;
         default: return -1;
     }
 }
 
 extern "C" float coarse_tolerance(double tol, double shift, double max_abs_mz) {
     return cs::coarse_tolerance(tol, shift, max_abs_mz);
-}
-
-// Number of (mz_r[k], mz_q[k], tol[k]) where the float32 bounds of f32_window
-// disagree with the float64 test exact_window((double)mz_r, (double)mz_q, tol, 0).
-extern "C" int f32_window_mismatches(const float *mz_r, const float *mz_q, const double *tol,
-                                     int n) {
-    int bad = 0;
-    for (int k = 0; k < n; ++k) {
-        float lo, hi;
-        cs::f32_window(mz_r[k], tol[k], lo, hi);
-        const bool f = (mz_q[k] >= lo) & (mz_q[k] <= hi);
-        bad += f != cs::exact_window((double)mz_r[k], (double)mz_q[k], tol[k], 0.0);
-    }
-    return bad;
 }

@@ -112,6 +112,33 @@ def test_uniform_256_vs_oracle(gpu, oracle):
     assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=0.1))
 
 
+@pytest.mark.parametrize("tol", [0.25, 0.5, 0.1, 0.3])
+def test_window_edges_on_a_grid(gpu, oracle, tol):
+    """m/z on a 0.25 grid (float32 numbers, the float32 kernel variant): with
+    tolerance 0.25 or 0.5 many peaks sit exactly on a window edge, where the
+    inclusive float64 test decides; 0.1 and 0.3 are not float32 numbers.
+    Integer-quarter intensities make equal products, so the tie order counts
+    too.  Two query tiles, the second partial."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(17)
+
+    def grid(n):
+        mzs, its = [], []
+        for _ in range(n):
+            k = int(rng.integers(0, 12))
+            mzs.append(np.sort(100 + 0.25 * rng.integers(0, 24, k)))
+            its.append(rng.integers(1, 4, k) / 4.0)
+        return PackedSpectra.from_arrays(mzs, its)
+
+    refs, ques = grid(29), grid(100)
+    say(f"grid edges tol {tol}")
+    want = oracle_csr(oracle, refs, ques, tolerance=tol)
+    assert want["n_candidates"].max() > 8  # some pairs take the wave path
+    for K in (1, 8):
+        assert_parity(device_run(refs, ques, tol=tol, K=K), want)
+
+
 def test_uniform_slice_fixture(gpu):
     """Same workload, first 32 x 32, against the committed golden fixture."""
     from cudams import synthetic

@@ -14,13 +14,6 @@ import pytest
 from cudams.spectra import PackedSpectra
 
 
-def f32_exact(refs, ques, shift, mzp):
-    """The F32 lane kernel's case: float32 m/z on both sides, no shift, mz_power 0."""
-    return (shift == 0.0 and mzp == 0.0
-            and all(np.array_equal(x.mz.astype(np.float32).astype(np.float64), x.mz)
-                    for x in (refs, ques)))
-
-
 def tile_run(L, K, refs, ques, tol, shift=0.0, mzp=0.0, inp=1.0):
     R, Q = refs.n_spectra, ques.n_spectra
     mx = float(max(np.abs(refs.mz).max(initial=0), np.abs(ques.mz).max(initial=0)))
@@ -32,7 +25,7 @@ def tile_run(L, K, refs, ques, tol, shift=0.0, mzp=0.0, inp=1.0):
                                              ques.intensities, ques.offsets)]
     rc = L.tile_scan(K, arr[0].ctypes.data, arr[1].ctypes.data, arr[2].ctypes.data, R,
                      arr[3].ctypes.data, arr[4].ctypes.data, arr[5].ctypes.data, Q, tol, shift,
-                     mzp, inp, mx, int(f32_exact(refs, ques, shift, mzp)), score.ctypes.data, m.ctypes.data, c.ctypes.data, h.ctypes.data)
+                     mzp, inp, mx, score.ctypes.data, m.ctypes.data, c.ctypes.data, h.ctypes.data)
     assert rc == 0
     return score, m, c, h.astype(bool)
 
@@ -117,58 +110,3 @@ def test_coarse_tolerance_bounds(tile_harness):
             t = tile_harness.coarse_tolerance(tol, 0.0, m)
             assert t > tol
             assert t <= tol + 16 * (m + tol) * 2.0 ** -24 + 1e-30 + 4 * np.spacing(np.float32(tol + 1e-30))
-
-
-def test_f32_window_matches_float64_test(tile_harness):
-    """The F32 lane kernel walks [f32_up(mz_r - tol), f32_down(mz_r + tol)]
-    and tests nothing else; that must equal the float64 test of matchms
-    find_matches (mz_r - tol <= mz_q <= mz_r + tol, bounds rounded in float64)
-    for every float32 m/z.  Checked where it can fail: query m/z on and one
-    float32 step either side of both float64 bounds (and of their float32
-    roundings), tolerances that are and are not float32 numbers, bounds that
-    cross zero."""
-    rng = np.random.default_rng(11)
-    tols = np.concatenate([[0.0, 0.01, 0.1, 0.5, 1.0, 2.0 ** -10, 1e-7, 3.0, 1e-30],
-                           rng.uniform(0, 1, 40), 10.0 ** rng.uniform(-8, 0, 40)])
-    mr = np.concatenate([np.float32([0.0, 1e-3, 0.005, 0.01, 50.0, 100.25, 999.99, 1000.0,
-                                     65504.0, 3e5]),
-                         rng.uniform(0, 2000, 200).astype(np.float32),
-                         (2.0 ** rng.integers(-3, 12, 50)).astype(np.float32)])
-    R, T, Qs = [], [], []
-    for t in tols:
-        for m in mr:
-            for b in (float(m) - t, float(m) + t):
-                c = np.float32(b)
-                for base in (c, np.nextafter(c, np.float32(-np.inf)), np.nextafter(c, np.float32(np.inf))):
-                    for q in (np.nextafter(base, np.float32(-np.inf)), base,
-                              np.nextafter(base, np.float32(np.inf))):
-                        R.append(m)
-                        T.append(t)
-                        Qs.append(q)
-    mr_a = np.array(R, np.float32)
-    mq_a = np.array(Qs, np.float32)
-    tol_a = np.array(T, np.float64)
-    # the float64 test must see both outcomes often, or the check pins nothing
-    inside = (mq_a.astype(np.float64) <= mr_a.astype(np.float64) + tol_a) & (
-        mq_a.astype(np.float64) >= mr_a.astype(np.float64) - tol_a)
-    assert 0.2 < inside.mean() < 0.8
-    bad = tile_harness.f32_window_mismatches(mr_a.ctypes.data, mq_a.ctypes.data, tol_a.ctypes.data,
-                                             len(mr_a))
-    assert bad == 0
-
-
-def test_f32_window_edges_are_exercised(tile_harness, oracle):
-    """The grid workload has peaks exactly on both window edges: the float32
-    window finds every oracle candidate there, and one a float32 step narrower
-    would lose some, so the grid check is sensitive to an off-by-one bound."""
-    rng = np.random.default_rng(3)
-    refs, ques = grid_spectra(rng, 23), grid_spectra(rng, 70)
-    assert f32_exact(refs, ques, 0.0, 0.0)
-    want = oracle.cosine_greedy_matrix((refs.mz, refs.intensities, refs.offsets),
-                                       (ques.mz, ques.intensities, ques.offsets), tolerance=0.25)
-    # tolerance 0.25 on a 0.25 grid: edge peaks are candidates
-    _, _, c, _ = tile_run(tile_harness, 8, refs, ques, 0.25)
-    np.testing.assert_array_equal(c, want["n_candidates"])
-    # the same walk with a tolerance one float32 step short of 0.25 misses them
-    _, _, c2, _ = tile_run(tile_harness, 8, refs, ques, float(np.nextafter(np.float32(0.25), 0)))
-    assert (c2 < want["n_candidates"]).any()
