@@ -471,8 +471,9 @@ __device__ __forceinline__ void wave_sync() {
 // Per-wave candidate pool of k_light: every exact candidate of the current
 // reference against the tile, in emit order, with its query lane (pq).  The
 // first kIdxRanks candidates of each query are also indexed by their rank
-// (pidx[rank][query], the rank is what the query's count atomic returns), so
-// a lane gathers its pair's candidates with independent loads.  A query
+// (pidx[rank][query], the rank is what the query's count atomic returns; one
+// spare row takes every later rank, so the store needs no branch), so a lane
+// gathers its pair's candidates with independent loads.  A query
 // contributes at most kMidCap entries (more means the pair is settled by the
 // heavy kernel anyway).
 constexpr int kPool = 128;
@@ -829,7 +830,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     // rank within the query and pool slot, both atomics in flight at once
                     const int rank = atomicAdd(&cnt[ql], 1);
                     const int pos = atomicAdd(pool_n, 1);
-                    if ((rank >= kMidCap) | (pos >= kPool)) {  // rare: out of line
+                    if ((rank >= kMidCap) | (pos >= kPool)) {  // rare
                         if (rank >= kMidCap) {  // heavy anyway: the slot holds no candidate
                             if (pos < kPool) pq[pos] = 0xffu;
                         } else {
