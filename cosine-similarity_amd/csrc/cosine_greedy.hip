@@ -551,15 +551,20 @@ __device__ __forceinline__ void sort_merge(double &v, uint32_t &ij, int lane) {
     if constexpr (J > 1) sort_merge<Kb, J / 2>(v, ij, lane);
 }
 
-// Bitonic sort of one (value, ij) per lane across the wave, descending by key.
-__device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij) {
+// Bitonic sort of one (value, ij) per lane across the wave, descending by key,
+// for keys in lanes [0, n) and -inf keys above (n wave-uniform).  Lanes
+// [0, 16) sorted by the 16-lane merges hold all n <= 16 keys in descending
+// order, so the wider merges run only for larger n.
+__device__ __forceinline__ void wave_sort_desc(double &v, uint32_t &ij, int n) {
     const int lane = threadIdx.x & (kWave - 1);
     sort_merge<2, 1>(v, ij, lane);
     sort_merge<4, 2>(v, ij, lane);
     sort_merge<8, 4>(v, ij, lane);
     sort_merge<16, 8>(v, ij, lane);
-    sort_merge<32, 16>(v, ij, lane);
-    sort_merge<64, 32>(v, ij, lane);
+    if (n > 16) {
+        sort_merge<32, 16>(v, ij, lane);
+        if (n > 32) sort_merge<64, 32>(v, ij, lane);
+    }
 }
 
 // s_waitcnt operand: vmcnt(0), expcnt and lgkmcnt not waited on (gfx9 layout:
@@ -636,7 +641,7 @@ __device__ __attribute__((noinline)) MidResult mid_settle(int wave, int lim, int
         ij = P.pij[e];
     }
     wave_sync();
-    wave_sort_desc(v, ij);
+    wave_sort_desc(v, ij, have);
     const int vi = (int)(ij >> 16), vj = (int)(ij & 0xffffu);
     bool alive = lane < have;
     MidResult r{0.0, 0};  // the same on every lane (read from lane f)
