@@ -1232,11 +1232,46 @@ __device__ __forceinline__ void heavy_lds_stage(double *bv, uint32_t *bij, int s
     }
 }
 
+// Two LDS stages at once, partner distances 2h then h (h >= 1) of bitonic
+// size `size` >= 4h, over the first `quads` groups of four entries x, x + h,
+// x + 2h, x + 3h (x with both stride bits clear): one load and at most one
+// store of each entry for the two stages.  An entry is stored only if it
+// moved, i.e. its ij changed: real entries have distinct ij (i, j < 65535)
+// and every -inf padding entry is (-inf, 0xffffffff).
+__device__ __forceinline__ void heavy_lds_stage2(double *bv, uint32_t *bij, int size, int h,
+                                                 int quads) {
+    for (int t = threadIdx.x; t < quads; t += kHeavyThreads) {
+        const int lowb = t & (h - 1);
+        const int x = lowb + 4 * (t - lowb);
+        const bool desc = (x & size) == 0;
+        int p[4];
+        double v[4];
+        uint32_t ij[4], ij0[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            p[k] = hphys(x + k * h);
+            v[k] = bv[p[k]];
+            ij[k] = ij0[k] = bij[p[k]];
+        }
+        reg_ce(v[0], ij[0], v[2], ij[2], desc);
+        reg_ce(v[1], ij[1], v[3], ij[3], desc);
+        reg_ce(v[0], ij[0], v[1], ij[1], desc);
+        reg_ce(v[2], ij[2], v[3], ij[3], desc);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (ij[k] != ij0[k]) {
+                bv[p[k]] = v[k];
+                bij[p[k]] = ij[k];
+            }
+        }
+    }
+}
+
 // Descending bitonic sort of the heavy buffer (entries at hphys), 256 threads.
 // With n2 = 256 E entries (E = 2 .. 16), the strides below E of every size
 // run in registers (one load and one store of each entry per size) and only
-// the larger strides go through LDS stage by stage: about 40% less LDS
-// traffic at 4096 entries.  Blocks wholly in the -inf padding are skipped.
+// the larger strides go through LDS two stages per pass.  Blocks wholly in
+// the -inf padding are skipped.
 template <int E>
 __device__ void heavy_sort_e(double *bv, uint32_t *bij, int n, int n2) {
     heavy_reg_pass<E>(bv, bij, 0, true, ((n + E - 1) / E) * E);
@@ -1244,7 +1279,12 @@ __device__ void heavy_sort_e(double *bv, uint32_t *bij, int n, int n2) {
     for (int size = 2 * E; size <= n2; size <<= 1) {
         const int live = ((n + size - 1) / size) * size;
         const int pairs = (live < n2 ? live : n2) >> 1;
-        for (int stride = size >> 1; stride >= E; stride >>= 1) {
+        int stride = size >> 1;
+        for (; stride >= 2 * E; stride >>= 2) {  // two strides per LDS pass
+            heavy_lds_stage2(bv, bij, size, stride >> 1, pairs >> 1);
+            __syncthreads();
+        }
+        if (stride >= E) {
             heavy_lds_stage(bv, bij, size, stride, pairs);
             __syncthreads();
         }
@@ -1257,9 +1297,9 @@ __device__ void sort_desc_heavy(double *bv, uint32_t *bij, int n) {
     const int tid = threadIdx.x;
     int n2 = 1;
     while (n2 < n) n2 <<= 1;
-    for (int e = n + tid; e < n2; e += kHeavyThreads) {
+    for (int e = n + tid; e < n2; e += kHeavyThreads) {  // no real entry has this ij
         bv[hphys(e)] = -__builtin_huge_val();
-        bij[hphys(e)] = 0u;
+        bij[hphys(e)] = 0xffffffffu;
     }
     __syncthreads();
     switch (n2 / kHeavyThreads) {
