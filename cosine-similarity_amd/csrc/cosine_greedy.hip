@@ -234,6 +234,7 @@ k_tile_sort(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t n
 // (value, lane, peak) among the bucket's elements.  Larger tiles are marked
 // in `big` and indexed by k_tile_sort + k_tile_dir.
 constexpr int kIndexThreads = 1024;
+static_assert(kIndexStage % kIndexThreads == 0, "k_tile_index holds a tile's peaks in registers");
 
 __global__ void __launch_bounds__(kIndexThreads)
 k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
@@ -261,8 +262,18 @@ k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t 
         __syncthreads();
         const float lo = tile_lo[g], iw = tile_iw[g];
         const int64_t g0 = loff[0];
-        for (int e = tid; e < np; e += kIndexThreads)
-            atomicAdd(&start[tile_bucket(mz[g0 + e] + shift32, lo, iw)], 1);
+        // the thread's peaks (e = tid + u * kIndexThreads), loaded once for
+        // the histogram and the scatter: one memory latency, not one per peak
+        constexpr int kPer = kIndexStage / kIndexThreads;
+        float xr[kPer];
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = tid + u * kIndexThreads;
+            xr[u] = e < np ? mz[g0 + e] + shift32 : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kPer; ++u)
+            if (tid + u * kIndexThreads < np) atomicAdd(&start[tile_bucket(xr[u], lo, iw)], 1);
         __syncthreads();
         // exclusive scan of start[0..kTileBuckets) (8 buckets per thread)
         int loc[kTileBuckets / kIndexThreads];
@@ -303,12 +314,15 @@ k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t 
         __syncthreads();
         int32_t *d = dir + g * (kTileBuckets + 1);
         for (int b = tid; b <= kTileBuckets; b += kIndexThreads) d[b] = start[b];
-        for (int e = tid; e < np; e += kIndexThreads) {  // scatter into buckets
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {  // scatter into buckets
+            const int e = tid + u * kIndexThreads;
+            if (e >= np) break;
             const int64_t gp = g0 + e;
             int ql = 0;  // last list whose offset is <= gp (binary search)
             for (int step = kTileQueries / 2; step > 0; step >>= 1)
                 if (ql + step < nl && loff[ql + step] <= gp) ql += step;
-            const float x = mz[gp] + shift32;
+            const float x = xr[u];
             const int pos = atomicAdd(&cursor[tile_bucket(x, lo, iw)], 1);
             bx[pos] = x;
             bw[pos] = tile_who(ql, (int)(gp - loff[ql]));
@@ -319,9 +333,24 @@ k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t 
             const float x = bx[e];
             const uint32_t w = bw[e];
             const int b = tile_bucket(x, lo, iw);
+            // rank = bucket start + entries of the bucket ordered before
+            // (x, w); eight loads in flight per round (a peak cluster puts
+            // about 100 entries in one bucket)
             int rank = start[b];
-            for (int k = start[b]; k < start[b + 1]; ++k)
-                rank += (bx[k] < x || (bx[k] == x && bw[k] < w)) ? 1 : 0;
+            int k = start[b];
+            const int kend = start[b + 1];
+            for (; k + 8 <= kend; k += 8) {
+                float xs[8];
+                uint32_t ws[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    xs[u] = bx[k + u];
+                    ws[u] = bw[k + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) rank += (xs[u] < x) | ((xs[u] == x) & (ws[u] < w));
+            }
+            for (; k < kend; ++k) rank += (bx[k] < x) | ((bx[k] == x) & (bw[k] < w));
             smz[pb + rank] = x;
             swho[pb + rank] = w;
         }
@@ -371,12 +400,29 @@ k_tile_values(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t
         const int np = tile_np[g];
         const int64_t pb = tile_pbase[g];
         int exact = 1;
-        for (int p = tid; p < np; p += blockDim.x) {
-            const uint32_t w = swho[pb + p];
-            const double x = inp[loff[w >> 16] + (w & 0xffffu)];
-            const float f = (float)x;
-            sint[pb + p] = f;
-            exact &= (double)f == x ? 1 : 0;
+        constexpr int kBatch = 8;  // loads in flight per thread
+        for (int p0 = tid; p0 < np; p0 += kBatch * (int)blockDim.x) {
+            uint32_t w[kBatch];
+            double x[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int p = p0 + u * (int)blockDim.x;
+                w[u] = p < np ? swho[pb + p] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int p = p0 + u * (int)blockDim.x;
+                x[u] = p < np ? inp[loff[w[u] >> 16] + (w[u] & 0xffffu)] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int p = p0 + u * (int)blockDim.x;
+                if (p < np) {
+                    const float f = (float)x[u];
+                    sint[pb + p] = f;
+                    exact &= (double)f == x[u] ? 1 : 0;
+                }
+            }
         }
         if (tid == 0) sint[pb + np] = 0.0f;  // the pad position
         exact = __syncthreads_and(exact);
