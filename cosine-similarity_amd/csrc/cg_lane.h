@@ -50,6 +50,13 @@ CS_HD bool exact_window(double mz_r, double mz_q, double tol, double shift) {
 
 // Up to K candidates kept sorted by key_greater (descending) in registers.
 // All indexing is compile-time after unrolling, so the arrays stay in VGPRs.
+// The unrolled steps past the wave's bound are left by an early exit for
+// K <= 8; for larger K a loop with an exit in the middle is not unrolled
+// (the arrays would go to scratch memory), so those steps are skipped one by
+// one instead.
+template <int K>
+constexpr bool kLaneEarlyExit = K <= 8;
+
 template <int K>
 struct LaneCandidates {
     double v[K];
@@ -63,7 +70,12 @@ struct LaneCandidates {
     CS_HD void insert(double cv, uint32_t cij, int bound = K) {
 #pragma unroll
         for (int s = 0; s < K; ++s) {
-            if (s >= bound) break;
+            if (s >= bound) {
+                if constexpr (kLaneEarlyExit<K>)
+                    break;
+                else
+                    continue;
+            }
             const bool take = (s >= n) | key_greater(cv, cij, v[s], ij[s]);
             const double tv = take ? v[s] : cv;
             const uint32_t tij = take ? ij[s] : cij;
@@ -84,7 +96,12 @@ struct LaneCandidates {
         bool acc[K];
 #pragma unroll
         for (int a = 0; a < K; ++a) {
-            if (a >= bound) break;
+            if (a >= bound) {
+                if constexpr (kLaneEarlyExit<K>)
+                    break;
+                else
+                    continue;
+            }
             bool ok = a < n;
             const uint32_t ia = ij[a] >> 16, ja = ij[a] & 0xffffu;
 #pragma unroll

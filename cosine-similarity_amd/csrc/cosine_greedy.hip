@@ -1015,7 +1015,12 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             cand.reset();
 #pragma unroll
             for (int s = 0; s < K; ++s) {  // independent gathers through the rank index
-                if (s >= bound) break;     // uniform
+                if (s >= bound) {  // uniform
+                    if constexpr (kLaneEarlyExit<K>)
+                        break;
+                    else
+                        continue;
+                }
                 if (small && s < n) {
                     const int e = pidx[s * kTileQueries + lane];
                     cand.insert(pv[e], pij[e], bound);
