@@ -1122,7 +1122,8 @@ k_light_batch(const LightArgs *jobs, const unsigned long long *items, int64_t n_
 // Heavy kernel: one workgroup per pair with more than 64 candidates
 // ---------------------------------------------------------------------------
 
-constexpr int kHeavyThreads = 256;
+constexpr int kHeavyThreads = 512;
+constexpr int kScanChunk = 256;    // pairs one workgroup checks per round of the overflow scan
 constexpr int kHeavyBuf = 4096;     // candidates one pair settles in a single pass
 // The heavy buffer keeps entry x at hphys(x) = x + x / 16: the sort's register
 // passes read 16 consecutive entries per thread, and the pad slot spreads a
@@ -1318,11 +1319,11 @@ __device__ __forceinline__ void heavy_lds_stage2(double *bv, uint32_t *bij, int 
     }
 }
 
-// Descending bitonic sort of the heavy buffer (entries at hphys), 256 threads.
-// With n2 = 256 E entries (E = 2 .. 16), the strides below E of every size
-// run in registers (one load and one store of each entry per size) and only
-// the larger strides go through LDS two stages per pass.  Blocks wholly in
-// the -inf padding are skipped.
+// Descending bitonic sort of the heavy buffer (entries at hphys) by the
+// kHeavyThreads threads.  With n2 = kHeavyThreads * E entries (E >= 2), the
+// strides below E of every size run in registers (one load and one store of
+// each entry per size) and only the larger strides go through LDS, two
+// stages per pass.  Blocks wholly in the -inf padding are skipped.
 template <int E>
 __device__ void heavy_sort_e(double *bv, uint32_t *bij, int n, int n2) {
     heavy_reg_pass<E>(bv, bij, 0, true, ((n + E - 1) / E) * E);
@@ -1360,7 +1361,7 @@ __device__ void sort_desc_heavy(double *bv, uint32_t *bij, int n) {
     case 2: heavy_sort_e<2>(bv, bij, n, n2); return;
     default: break;
     }
-    // n2 <= 256: every stage through LDS
+    // n2 <= kHeavyThreads: every stage through LDS
     for (int size = 2; size <= n2; size <<= 1) {
         const int live = ((n + size - 1) / size) * size;
         const int pairs = (live < n2 ? live : n2) >> 1;
@@ -1600,7 +1601,7 @@ __device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
     __shared__ int bn_s;
     __shared__ unsigned long long item_s;
     __shared__ int nlist;
-    __shared__ long long list[kHeavyThreads];
+    __shared__ long long list[kScanChunk];
     // counters[4] counts the big pairs, queued from the top of the queue
     // downwards; they are taken first so they do not form the tail
     const unsigned long long count = a.counters[0], big = a.counters[4];
@@ -1622,11 +1623,11 @@ __device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
         for (int jb = 0; jb < n_jobs; ++jb) {
             const HeavyArgs &J = jobs ? jobs[jb] : a;
             const int64_t total = J.nR * J.nQ;
-            for (int64_t c = blockIdx.x; c * kHeavyThreads < total; c += gridDim.x) {
+            for (int64_t c = blockIdx.x; c * kScanChunk < total; c += gridDim.x) {
                 if (tid == 0) nlist = 0;
                 __syncthreads();
-                const int64_t p = c * kHeavyThreads + tid;
-                if (p < total) {
+                const int64_t p = c * kScanChunk + tid;
+                if (tid < kScanChunk && p < total) {
                     const int64_t r = p / J.nQ, q = p - r * J.nQ;
                     if (J.matches[r * J.ld + q] == -1) list[atomicAdd(&nlist, 1)] = p;
                 }
