@@ -6,7 +6,7 @@
 // Pipeline (all asynchronous on the caller's stream, no host sync):
 //   1. k_fill_u64        zero the counters (a kernel, not a memset node, so the
 //                        sequence can sit in a captured graph)
-//   2. k_tile_meta, k_tile_index, k_tile_sort, k_tile_dir, k_tile_values
+//   2. k_tile_meta, k_tile_index, k_tile_sort, k_tile_values
 //                        per 64-query tile: every peak's m/z in one sorted
 //                        list with its (query lane, peak) and a bucket
 //                        directory over the tile's m/z range
@@ -192,9 +192,13 @@ __device__ void tile_rank_scatter(X xv, const int64_t *loff, int nl, int np, int
     }
 }
 
+// Tiles above kIndexStage peaks: the list by rank merge, then its directory,
+// dir[b] = first list position whose bucket is >= b for b in [0, kTileBuckets]
+// (read back from the list this workgroup has just written).
 __global__ void __launch_bounds__(256)
 k_tile_sort(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
-            const int32_t *tile_np, const int64_t *tile_pbase, float *smz, uint32_t *swho) {
+            const int32_t *tile_np, const int64_t *tile_pbase, const float *tile_lo,
+            const float *tile_iw, float *smz, uint32_t *swho, int32_t *dir) {
     __shared__ int64_t loff[kTileQueries + 1];
     __shared__ float xs[kSortStage];
     for (int64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
@@ -225,6 +229,15 @@ k_tile_sort(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t n
             swho[pb + np] = 0u;
         }
         __syncthreads();
+        const float *s = smz + pb;
+        const float lo = tile_lo[g], iw = tile_iw[g];
+        int32_t *d = dir + g * (kTileBuckets + 1);
+        for (int p = threadIdx.x; p <= np; p += blockDim.x) {
+            const int bp = p < np ? tile_bucket(s[p], lo, iw) : kTileBuckets;
+            const int bq = p > 0 ? tile_bucket(s[p - 1], lo, iw) : -1;
+            for (int b = bq + 1; b <= bp; ++b) d[b] = p;
+        }
+        __syncthreads();
     }
 }
 
@@ -232,7 +245,7 @@ k_tile_sort(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t n
 // bucket (the exclusive prefix of the bucket histogram is the directory),
 // then each element's final place inside its bucket is its rank by
 // (value, lane, peak) among the bucket's elements.  Larger tiles are marked
-// in `big` and indexed by k_tile_sort + k_tile_dir.
+// in `big` and indexed by k_tile_sort.
 constexpr int kIndexThreads = 1024;
 static_assert(kIndexStage % kIndexThreads == 0, "k_tile_index holds a tile's peaks in registers");
 
@@ -359,24 +372,6 @@ k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t 
             swho[pb + np] = 0u;
         }
         __syncthreads();
-    }
-}
-
-// dir[b] = first list position whose bucket is >= b, for b in [0, kTileBuckets].
-__global__ void __launch_bounds__(256)
-k_tile_dir(int64_t ntiles, const int32_t *tile_np, const int64_t *tile_pbase,
-           const float *tile_lo, const float *tile_iw, const float *smz, int32_t *dir) {
-    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int np = tile_np[t];
-        if (np <= kIndexStage) continue;
-        const float *s = smz + tile_pbase[t];
-        const float lo = tile_lo[t], iw = tile_iw[t];
-        int32_t *d = dir + t * (kTileBuckets + 1);
-        for (int p = threadIdx.x; p <= np; p += blockDim.x) {
-            const int bp = p < np ? tile_bucket(s[p], lo, iw) : kTileBuckets;
-            const int bq = p > 0 ? tile_bucket(s[p - 1], lo, iw) : -1;
-            for (int b = bq + 1; b <= bp; ++b) d[b] = p;
-        }
     }
 }
 
@@ -2059,11 +2054,9 @@ cs_status launch_prepare(const PrepJob &one, const PrepJob *jobs, const int32_t 
                        L.smz, L.swho, L.dir);
     CS_LAUNCH_CHECK("k_tile_index");
     hipLaunchKernelGGL(k_tile_sort, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, one, jobs,
-                       tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.smz, L.swho);
+                       tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz,
+                       L.swho, L.dir);
     CS_LAUNCH_CHECK("k_tile_sort");
-    hipLaunchKernelGGL(k_tile_dir, dim3(blocks(L.ntiles, 1)), dim3(256), 0, hs, L.ntiles,
-                       L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz, L.dir);
-    CS_LAUNCH_CHECK("k_tile_dir");
     hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1)), dim3(1024), 0, hs, one, jobs,
                        tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.swho, L.sint,
                        L.tile_flags);
