@@ -1340,10 +1340,79 @@ __device__ void heavy_sort_e(double *bv, uint32_t *bij, int n, int n2) {
     }
 }
 
+// One wave sorts n2 = 64 * EPL entries of the heavy buffer descending, in
+// registers: lane l holds entries EPL * l .. EPL * l + EPL - 1.  Strides below
+// EPL are exchanges inside a lane; a stride s >= EPL pairs lanes at distance
+// s / EPL (DPP, swizzle or bpermute moves).  No workgroup barrier: a small
+// pair's sort costs one wave a few thousand instructions instead of the whole
+// workgroup a barrier per stage.
+template <int EPL, int SIZE, int STRIDE>
+__device__ __forceinline__ void wave_heavy_stage(double (&v)[EPL], uint32_t (&ij)[EPL], int lane) {
+    if constexpr (STRIDE >= EPL) {
+        constexpr int D = STRIDE / EPL;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int x = EPL * lane + k;
+            const bool desc = (x & SIZE) == 0;
+            const bool lower = (x & STRIDE) == 0;
+            const double ov = xor_lane_double<D>(v[k]);
+            const uint32_t oij = (uint32_t)xor_lane<D>((int)ij[k]);
+            // the lower entry of a descending block keeps the greater key
+            const bool take = (lower == desc) ? key_greater(ov, oij, v[k], ij[k])
+                                              : key_greater(v[k], ij[k], ov, oij);
+            v[k] = take ? ov : v[k];
+            ij[k] = take ? oij : ij[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            if (k & STRIDE) continue;
+            reg_ce(v[k], ij[k], v[k + STRIDE], ij[k + STRIDE], ((EPL * lane + k) & SIZE) == 0);
+        }
+    }
+    if constexpr (STRIDE > 1) wave_heavy_stage<EPL, SIZE, STRIDE / 2>(v, ij, lane);
+}
+template <int EPL, int SIZE>
+__device__ __forceinline__ void wave_heavy_sizes(double (&v)[EPL], uint32_t (&ij)[EPL], int lane) {
+    wave_heavy_stage<EPL, SIZE, SIZE / 2>(v, ij, lane);
+    if constexpr (SIZE < kWave * EPL) wave_heavy_sizes<EPL, SIZE * 2>(v, ij, lane);
+}
+template <int EPL>
+__device__ __attribute__((noinline)) void wave_sort_heavy(double *bv, uint32_t *bij, int n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    double v[EPL];
+    uint32_t ij[EPL];
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+        const int x = EPL * lane + k;
+        v[k] = x < n ? bv[hphys(x)] : -__builtin_huge_val();
+        ij[k] = x < n ? bij[hphys(x)] : 0xffffffffu;
+    }
+    wave_heavy_sizes<EPL, 2>(v, ij, lane);
+#pragma unroll
+    for (int k = 0; k < EPL; ++k) {
+        bv[hphys(EPL * lane + k)] = v[k];
+        bij[hphys(EPL * lane + k)] = ij[k];
+    }
+    wave_sync();
+}
+
+// Sorts the heavy buffer's first n entries descending.  From 128 to 512
+// entries wave 0 alone sorts in registers (the other waves return at once);
+// larger buffers are sorted by the whole workgroup.
 __device__ void sort_desc_heavy(double *bv, uint32_t *bij, int n) {
     const int tid = threadIdx.x;
     int n2 = 1;
     while (n2 < n) n2 <<= 1;
+    if (n2 >= 2 * kWave && n2 <= 4 * kWave) {  // uniform
+        if (tid < kWave) {
+            if (n2 == 2 * kWave)
+                wave_sort_heavy<2>(bv, bij, n);
+            else
+                wave_sort_heavy<4>(bv, bij, n);
+        }
+        return;
+    }
     for (int e = n + tid; e < n2; e += kHeavyThreads) {  // no real entry has this ij
         bv[hphys(e)] = -__builtin_huge_val();
         bij[hphys(e)] = 0xffffffffu;
@@ -1463,8 +1532,8 @@ __device__ void write_error(const HeavyArgs &a, int64_t o) {
 
 // One pair, whole block: stage both spectra, enumerate every candidate once
 // into LDS, sort, greedy.  More than kHeavyBuf candidates: queued as giant.
-__device__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q, unsigned char *lds,
-                           int *bn_s) {
+__device__ __forceinline__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q,
+                                           unsigned char *lds, int *bn_s) {
     const int tid = threadIdx.x, nt = blockDim.x;
     const SpecView &R = a.R, &Q = a.Q;
     const int64_t roff = R.off[r], qoff = Q.off[q];
@@ -1638,9 +1707,14 @@ __device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
     }
 }
 
-__global__ void __launch_bounds__(kHeavyThreads) k_heavy(HeavyArgs a) { heavy_body(a, nullptr, 1); }
+// Two workgroups per compute unit (the LDS allows two): at most 128 VGPRs.
+__global__ void __launch_bounds__(kHeavyThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_heavy(HeavyArgs a) {
+    heavy_body(a, nullptr, 1);
+}
 
-__global__ void __launch_bounds__(kHeavyThreads) k_heavy_batch(const HeavyArgs *jobs, int n_jobs) {
+__global__ void __launch_bounds__(kHeavyThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_heavy_batch(const HeavyArgs *jobs, int n_jobs) {
     heavy_body(jobs[0], jobs, n_jobs);
 }
 
