@@ -1397,9 +1397,10 @@ __device__ __attribute__((noinline)) void wave_sort_heavy(double *bv, uint32_t *
     wave_sync();
 }
 
-// Sorts the heavy buffer's first n entries descending.  From 128 to 512
-// entries wave 0 alone sorts in registers (the other waves return at once);
-// larger buffers are sorted by the whole workgroup.
+// Sorts the heavy buffer's first n entries descending.  Padded to 128 or 256
+// entries, wave 0 alone sorts in registers (the other waves return at once);
+// larger buffers are sorted by the whole workgroup (for 512 entries one wave
+// was no faster).
 __device__ void sort_desc_heavy(double *bv, uint32_t *bij, int n) {
     const int tid = threadIdx.x;
     int n2 = 1;
