@@ -1095,10 +1095,13 @@ __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
 
 // A batch: a persistent grid takes work items (job << 48 | reference block
 // << 24 | tile), costliest first, from a shared counter until none is left.
+// The job table and item list are read-only and alias nothing the kernel
+// writes (__restrict__): without that the compiler reloads job fields after
+// every store, about 20% of the kernel's time.
 template <int K, bool F32>
 __global__ void __launch_bounds__(kLightThreads)
-k_light_batch(const LightArgs *jobs, const unsigned long long *items, int64_t n_items,
-              unsigned long long *next) {
+k_light_batch(const LightArgs *__restrict__ jobs, const unsigned long long *__restrict__ items,
+              int64_t n_items, unsigned long long *next) {
     __shared__ unsigned long long item_s;
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(next, 1ull);
@@ -1715,7 +1718,7 @@ k_heavy(HeavyArgs a) {
 }
 
 __global__ void __launch_bounds__(kHeavyThreads) __attribute__((amdgpu_waves_per_eu(4)))
-k_heavy_batch(const HeavyArgs *jobs, int n_jobs) {
+k_heavy_batch(const HeavyArgs *__restrict__ jobs, int n_jobs) {
     heavy_body(jobs[0], jobs, n_jobs);
 }
 
@@ -1879,7 +1882,8 @@ __device__ void giant_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
 
 __global__ void __launch_bounds__(kWave) k_giant(HeavyArgs a) { giant_body(a, nullptr, 1); }
 
-__global__ void __launch_bounds__(kWave) k_giant_batch(const HeavyArgs *jobs, int n_jobs) {
+__global__ void __launch_bounds__(kWave)
+k_giant_batch(const HeavyArgs *__restrict__ jobs, int n_jobs) {
     giant_body(jobs[0], jobs, n_jobs);
 }
 
