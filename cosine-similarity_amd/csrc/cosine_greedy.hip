@@ -886,10 +886,8 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     }
                     if constexpr (F32 && TF)  // ((1 * int_pow_r) * 1) * int_pow_q
                         pv[pos] = rin * (double)qin;
-                    else if constexpr (F32)
-                        pv[pos] = rin * Q.inp[qoff_l[ql] + j];
-                    else
-                        pv[pos] = pair_value(R, roff + i, Q, qoff_l[ql] + j);
+                    // otherwise the value needs the query peak's arrays in
+                    // HBM: filled for the whole pool after the probe
                     pij[pos] = pack_ij(i, j);
                     pq[pos] = (unsigned char)ql;
                     // ranks past the index go to a spare row nobody reads
@@ -939,6 +937,32 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                 const float ri = i < nr ? R.mz[roff + i] : 0.0f;
                 const double rin = i < nr ? R.inp[roff + i] : 0.0;
                 chunk(ib, ri, rin);
+            }
+            if constexpr (!(F32 && TF)) {
+                // candidate values from HBM, every pool entry at once (two
+                // per lane, loads in flight together) instead of one memory
+                // round trip per emit round
+                static_assert(kPool <= 2 * kWave, "two pool entries per lane");
+                wave_sync();
+                const int used = uniform(*pool_n) < kPool ? uniform(*pool_n) : kPool;
+                int e[2];
+                bool has[2];
+                double v[2];
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    e[h] = lane + h * kWave;
+                    const int ql = e[h] < used ? (int)pq[e[h]] : 0xff;
+                    has[h] = ql != 0xff;
+                    const uint32_t ij = has[h] ? pij[e[h]] : 0u;
+                    const int i = (int)(ij >> 16), j = (int)(ij & 0xffffu);
+                    if constexpr (F32)
+                        v[h] = has[h] ? R.inp[roff + i] * Q.inp[qoff_l[ql] + j] : 0.0;
+                    else
+                        v[h] = has[h] ? pair_value(R, roff + i, Q, qoff_l[ql] + j) : 0.0;
+                }
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    if (has[h]) pv[e[h]] = v[h];
             }
         };
         auto queue_heavy = [&](int n) {
