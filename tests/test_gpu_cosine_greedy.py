@@ -322,6 +322,34 @@ def test_tiles_larger_than_lds(gpu, oracle):
     assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=0.05))
 
 
+@pytest.mark.parametrize("tol", [0.05, 0.5])
+def test_spectra_above_staging_size(gpu, oracle, tol):
+    """Spectra of 1100-1400 peaks (above the 1024 the cooperative kernel
+    stages in LDS): its enumeration reads the spectra from HBM and its greedy
+    runs without the staging-area tables.  Tolerance 0.05 gives hundreds of
+    candidates per pair (heavy kernel), 0.5 thousands (giant kernel too)."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(23)
+
+    def spectra(n):
+        mzs, its = [], []
+        for _ in range(n):
+            k = int(rng.integers(1100, 1400))
+            mzs.append(np.sort(rng.uniform(100, 300, k).astype(np.float32)).astype(np.float64))
+            its.append(rng.random(k) + 0.01)
+        return PackedSpectra.from_arrays(mzs, its)
+
+    refs, ques = spectra(5), spectra(7)
+    assert max(np.diff(refs.offsets).max(), np.diff(ques.offsets).max()) > 1024
+    say(f"spectra above staging size, tol {tol}")
+    want = oracle_csr(oracle, refs, ques, tolerance=tol)
+    assert want["n_candidates"].min() > 64  # every pair leaves the lane kernel
+    if tol == 0.5:
+        assert want["n_candidates"].max() > 4096  # some go on to the giant kernel
+    assert_parity(device_run(refs, ques, tol=tol), want)
+
+
 def test_stage_reuse_with_other_references(gpu, oracle):
     """PREPARE once for a query set, then LIGHT|HEAVY with two different
     reference sets on the same workspace."""
