@@ -248,6 +248,35 @@ def test_empty_and_ragged(gpu, oracle):
     assert_parity(res, oracle_csr(oracle, p, p, tolerance=0.1))
 
 
+@pytest.mark.parametrize("K", [1, 8])
+def test_zero_intensities(gpu, oracle, K):
+    """Peaks of intensity 0 (candidate value 0: ties broken by (i, j) alone)
+    and one all-zero spectrum per side, whose zero norm makes every pair
+    with a candidate 0/0 = NaN, as in matchms; dense enough at tol 0.5 for
+    the wave and heavy paths."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(41)
+
+    def spectra(n, seed_shift):
+        mzs, its = [], []
+        for x in range(n):
+            k = int(rng.integers(3, 120))
+            mz = np.sort(rng.uniform(100, 200, k).astype(np.float32)).astype(np.float64)
+            it = rng.random(k) * (rng.random(k) > 0.3)
+            if x == seed_shift:
+                it = np.zeros(k)
+            mzs.append(mz)
+            its.append(it)
+        return PackedSpectra.from_arrays(mzs, its)
+
+    refs, ques = spectra(21, 4), spectra(70, 9)
+    say(f"zero intensities K {K}")
+    want = oracle_csr(oracle, refs, ques, tolerance=0.5)
+    assert np.isnan(want["score"]).any() and (want["n_candidates"] > 64).any()
+    assert_parity(device_run(refs, ques, tol=0.5, K=K), want)
+
+
 def test_overflow_flag_rule(gpu, oracle):
     from cudams import synthetic
 
