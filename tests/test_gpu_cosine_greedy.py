@@ -112,11 +112,12 @@ def test_uniform_256_vs_oracle(gpu, oracle):
     assert_parity(res, oracle_csr(oracle, refs, ques, tolerance=0.1))
 
 
-@pytest.mark.parametrize("tol", [0.25, 0.5, 0.1, 0.3])
+@pytest.mark.parametrize("tol", [0.25, 0.5, 0.1, 0.3, 0.0])
 def test_window_edges_on_a_grid(gpu, oracle, tol):
     """m/z on a 0.25 grid (float32 numbers, the float32 kernel variant): with
     tolerance 0.25 or 0.5 many peaks sit exactly on a window edge, where the
-    inclusive float64 test decides; 0.1 and 0.3 are not float32 numbers.
+    inclusive float64 test decides; 0.1 and 0.3 are not float32 numbers;
+    tolerance 0 matches equal m/z only.
     Integer-quarter intensities make equal products, so the tie order counts
     too.  Two query tiles, the second partial."""
     from cudams.spectra import PackedSpectra
@@ -134,7 +135,7 @@ def test_window_edges_on_a_grid(gpu, oracle, tol):
     refs, ques = grid(29), grid(100)
     say(f"grid edges tol {tol}")
     want = oracle_csr(oracle, refs, ques, tolerance=tol)
-    assert want["n_candidates"].max() > 8  # some pairs take the wave path
+    assert want["n_candidates"].max() > (1 if tol == 0.0 else 8)  # wave path above 8
     for K in (1, 8):
         assert_parity(device_run(refs, ques, tol=tol, K=K), want)
 
