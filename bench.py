@@ -47,76 +47,106 @@ def traffic_from_profiles(kernel: str, workload: str):
     return None if e is None else e.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline_cosine(packed, tol, seconds_target: float):
-    """The oracle (C restatement of matchms CosineGreedy, OpenMP over rows) on a
-    stripe of reference rows against all queries, on this host's cores."""
+def host_threads() -> int:
+    """Host threads for the CPU legs: the cores this process may run on, capped
+    by OMP_NUM_THREADS when the environment sets it (a GPU box allots each GPU
+    a share of its cores that way)."""
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return cores
+
+
+def cpu_baseline_cosine(packed, tol):
+    """The oracle (C restatement of matchms CosineGreedy, float64, OpenMP over
+    rows) on the whole matrix, on this host's cores.  Returns the baseline
+    record and the oracle's matrix (used for the parity fields)."""
+    from oracle import oracle
+
+    cores = host_threads()
+    n = packed.n_spectra
+    csr = (packed.mz, packed.intensities, packed.offsets)
+    t0 = time.perf_counter()
+    want = oracle.cosine_greedy_matrix(csr, csr, tolerance=tol, nthreads=cores)
+    t = time.perf_counter() - t0
+    # one thread: the per-core rate, on a stripe of rows sized to about 2 s
+    rows1 = max(1, min(n, int(n * n / t / cores * 2.0 / n)))
+    sub1 = (packed.mz[: packed.offsets[rows1]], packed.intensities[: packed.offsets[rows1]],
+            packed.offsets[: rows1 + 1])
+    t1 = time.perf_counter()
+    oracle.cosine_greedy_matrix(sub1, csr, tolerance=tol, nthreads=1)
+    t1 = time.perf_counter() - t1
+    rec = {"value": n * n / t, "unit": "pairs/s", "cores": cores, "kind": "port",
+           "host_cores_visible": len(os.sched_getaffinity(0)),
+           "sample": f"all {n} x {n} pairs ({n * n}), {t:.1f} s, oracle/cs_oracle.c float64, "
+                     f"OpenMP over reference rows",
+           "single_core": {"value": rows1 * n / t1, "unit": "pairs/s", "cores": 1,
+                           "sample": f"first {rows1} reference rows x all {n} queries, "
+                                     f"{t1:.1f} s"}}
+    return rec, want
+
+
+def cosine_parity(got: dict, want: dict, packed, tol, match_limit) -> dict:
+    """GPU matrix against the oracle's: mismatch counts, the largest score
+    error, and near ties.  ``near_tie_pairs`` counts the pairs whose accepted
+    peak pairs change when candidate values are rounded to float32 (the
+    reference GPU kernel's stored products); ``near_tie_reassignments`` counts
+    the pairs where the GPU result departs from the float64 oracle: matches
+    differ, or the score is more than two float32 ulps from the oracle's."""
     import numpy as np
 
     from oracle import oracle
 
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
-    n = packed.n_spectra
+    err = np.abs(got["score"].astype(np.float64) - want["score"])
+    ulp = np.spacing(np.abs(want["score"]).astype(np.float32)).astype(np.float64)
+    mism = got["matches"] != want["matches"]
+    reassigned = mism | (err > 2.0 * ulp)
     csr = (packed.mz, packed.intensities, packed.offsets)
-
-    def stripe(rows):
-        off = packed.offsets
-        sub = (packed.mz[: off[rows]], packed.intensities[: off[rows]], off[: rows + 1])
-        t0 = time.perf_counter()
-        oracle.cosine_greedy_matrix(sub, csr, tolerance=tol, nthreads=cores)
-        return time.perf_counter() - t0
-
-    rows = 8
-    t = stripe(rows)
-    while t < 0.5 and rows < n:
-        rows = min(n, rows * 4)
-        t = stripe(rows)
-    rate = rows * n / t
-    want = int(min(n, max(rows, rate * seconds_target / n)))
-    if want > rows:
-        rows = want
-        t = stripe(rows)
-        rate = rows * n / t
-    # one core: the per-core rate, on a stripe sized to about 2 s
-    rows1 = max(1, min(n, int(rate / cores * 2.0 / n)))
-    t1 = time.perf_counter()
-    sub1 = (packed.mz[: packed.offsets[rows1]], packed.intensities[: packed.offsets[rows1]],
-            packed.offsets[: rows1 + 1])
-    oracle.cosine_greedy_matrix(sub1, csr, tolerance=tol, nthreads=1)
-    t1 = time.perf_counter() - t1
-    return {"value": rate, "unit": "pairs/s", "cores": cores, "kind": "port",
-            "sample": f"first {rows} reference rows x all {n} queries ({rows * n} pairs), "
-                      f"{t:.1f} s, oracle/cs_oracle.c float64, OpenMP",
-            "single_core": {"value": rows1 * n / t1, "unit": "pairs/s", "cores": 1,
-                            "sample": f"first {rows1} reference rows x all {n} queries, "
-                                      f"{t1:.1f} s"}}
+    ties = oracle.near_tie_pairs(csr, csr, tolerance=tol, nthreads=host_threads()).astype(bool)
+    return {"pairs": int(err.size), "oracle": "oracle/cs_oracle.c float64, every pair",
+            "matches_mismatch": int(mism.sum()),
+            "overflow_mismatch": int((got["overflow"] != (want["n_candidates"] >= match_limit)).sum()),
+            "max_abs_score_err": float(err.max()), "score_atol": 1e-5,
+            "near_tie_pairs": int(ties.sum()),
+            "near_tie_reassignments": int(reassigned.sum()),
+            "near_tie_reassignments_among_near_tie_pairs": int((reassigned & ties).sum()),
+            "overflow_pairs": int(want["overflow"].sum())}
 
 
-def cpu_baseline_fingerprint(fa, fb, seconds_target: float):
-    """The oracle's float64 fingerprint restatement (jaccard, OpenMP over rows)
-    on a stripe of reference rows against all queries."""
-    from oracle import oracle
+def light_roofline(n, lens, light_ms) -> dict:
+    """Roofline of k_light (the LIGHT stage) against HBM.  Algorithmic bytes
+    per launch are the compulsory ones: the 9 output bytes of every pair
+    (score f32 + matches i32 + overflow u8) and each spectrum's peaks read
+    once (float32 m/z + float32 intensity, 8 B per peak per side).  The
+    streaming model that re-reads both spectra for every pair,
+    8 (n_r + n_q) + 9 bytes per pair, is kept as ``effective_gbs``."""
+    peaks = float(lens.sum())
+    pairs = float(n) * n
+    compulsory = 9.0 * pairs + 8.0 * (peaks + peaks)
+    streaming = 8.0 * (n * peaks + n * peaks) + 9.0 * pairs
+    t = light_ms / 1e3
+    achieved = compulsory / t / 1e9
+    rec = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS,
+           "traffic": traffic_from_profiles("k_light", f"cosine_{n}"),
+           "kernel": "k_light", "kernel_ms": light_ms, "algorithmic_bytes": compulsory,
+           "effective_gbs": streaming / t / 1e9,
+           "effective_note": "8 (n_r + n_q) + 9 bytes per pair, every spectrum re-read per pair"}
+    valu = profile_field("k_light", f"cosine_{n}", "valu_busy")
+    if valu is not None:
+        rec["limiter"] = "valu_issue"
+        rec["valu_frac"] = valu
+    return rec
 
-    cores = min(len(os.sched_getaffinity(0)), 16)
-    n = fa.shape[0]
 
-    def stripe(rows):
-        t0 = time.perf_counter()
-        oracle.fingerprint_matrix(fa[:rows], fb, "jaccard", nthreads=cores)
-        return time.perf_counter() - t0
-
-    rows = 4
-    t = stripe(rows)
-    while t < 0.3 and rows < n:
-        rows = min(n, rows * 4)
-        t = stripe(rows)
-    want = int(min(n, max(rows, rows * seconds_target / t)))
-    if want > rows:
-        rows = want
-        t = stripe(rows)
-    return {"value": rows * fb.shape[0] / t, "unit": "pairs/s", "cores": cores, "kind": "port",
-            "sample": f"first {rows} reference rows x all {fb.shape[0]} queries, {t:.1f} s, "
-                      f"oracle/cs_oracle.c fingerprint_matrix (jaccard), OpenMP"}
+def profile_field(kernel: str, workload: str, field: str):
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        e = json.load(fh).get(f"{kernel}@{workload}")
+    return None if e is None else e.get(field)
 
 
 def bench_cosine(a) -> dict:
@@ -172,9 +202,6 @@ def bench_cosine(a) -> dict:
 
     lens = np.diff(packed.offsets).astype(np.float64)
     pairs = n * n
-    alg_bytes = 8.0 * (n * lens.sum() + n * lens.sum()) + 9.0 * pairs
-    light_s = stage_ms["light"] / 1e3
-    achieved = alg_bytes / light_s / 1e9
     res = {
         "metric": "CosineGreedy pairs/s at 8k x 8k; p50/p99 search latency served vs plain",
         "value": pairs / (ms / 1e3),
@@ -193,19 +220,39 @@ def bench_cosine(a) -> dict:
                    "match_limit": ml, "peaks": "round(lognormal(ln 60, 0.8)) in [5, 1024]",
                    "cluster_fraction": 0.2, "seed": 5, "mean_peaks": float(lens.mean())},
         "stage_ms": stage_ms,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles("k_light", f"cosine_{n}")},
+        "roofline": light_roofline(n, lens, stage_ms["light"]),
     }
+    got = {"score": score.cpu().numpy(), "matches": matches.cpu().numpy(),
+           "overflow": overflow.cpu().numpy()}
+    del score, matches, overflow
+    want = None
     if not a.no_cpu_baseline:
-        log("cpu baseline")
-        res["cpu_baseline"] = cpu_baseline_cosine(packed, tol, a.cpu_seconds)
+        log("cpu baseline (the oracle over the whole matrix)")
+        res["cpu_baseline"], want = cpu_baseline_cosine(packed, tol)
     else:
         res["cpu_baseline"] = None
+    if not a.no_parity:
+        if want is None:
+            from oracle import oracle
+
+            csr = (packed.mz, packed.intensities, packed.offsets)
+            want = oracle.cosine_greedy_matrix(csr, csr, tolerance=tol, nthreads=host_threads())
+        log("parity against the oracle")
+        res["parity"] = cosine_parity(got, want, packed, tol, ml)
+        log(f"parity {res['parity']}")
+    del got, want
+    if not a.no_matrix_wall:
+        res["matrix_wall"] = bench_matrix_wall(a, packed)
     return res
 
 
-def bench_fingerprint(a) -> dict:
+def bench_fingerprint_measures(a) -> dict:
+    """Jaccard, Dice and cosine on 4096 x 4096 fingerprints of 2048 bits
+    (seeds 3 and 4, per-row density U[0.02, 0.3]).  Per measure: device time
+    per call from HIP events on the launch stream, the matrix-core roofline
+    (2 R Q K bit operations per call against the dense fp4 peak the kernel
+    runs at, and against the int8 peak), the oracle on the host cores, and
+    parity against it over the whole matrix."""
     import numpy as np
     import torch
 
@@ -220,31 +267,90 @@ def bench_fingerprint(a) -> dict:
     wb = torch.from_numpy(synthetic.pack_bits(fb).view(np.int32)).to(dev)
     out = torch.empty((n, n), dtype=torch.float32, device=dev)
     ws = Workspace(dev)
-    for _ in range(a.warmup):
-        fingerprint_similarity("jaccard", wa, wb, out, workspace=ws)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        fingerprint_similarity("jaccard", wa, wb, out, workspace=ws)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    stream = torch.cuda.current_stream()
     ops = 2.0 * n * n * bits
-    res = {
-        "metric": "FingerprintSimilarity jaccard pairs/s at 4096 x 4096 x 2048 bits",
-        "value": n * n / (ms / 1e3), "unit": "pairs/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp4", "data": "synthetic",
-        "config": {"workload": "fingerprint_4096x4096x2048_jaccard"},
-        "roofline": {"bound": "mfma", "achieved": ops / (ms / 1e3) / 1e12, "peak": FP4_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": ops / (ms / 1e3) / 1e12 / FP4_PEAK_TFLOPS,
-                     "traffic": None,
-                     # the same call against the int8 MFMA peak, and the
-                     # float32 output write against HBM
-                     "int8_frac": ops / (ms / 1e3) / 1e12 / INT8_PEAK_TOPS,
-                     "output_hbm_frac": 4.0 * n * n / (ms / 1e3) / 8.0e12},
-        "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline_fingerprint(fa, fb, 5.0),
-    }
+    res = {"config": {"workload": "fingerprint_4096x4096x2048", "refs": n, "queries": n,
+                      "bits": bits, "density": "per-row U[0.02, 0.3]", "seeds": [3, 4]},
+           "dtype": "fp4 (0/1 bit planes, exact float32 counts)", "measures": {}}
+    steps = max(a.steps, 10)
+    for measure in ("jaccard", "dice", "cosine"):
+        for _ in range(max(a.warmup, 1)):
+            fingerprint_similarity(measure, wa, wb, out, workspace=ws)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fingerprint_similarity(measure, wa, wb, out, workspace=ws)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        tflops = ops / (ms / 1e3) / 1e12
+        m = {"ms_per_call": ms, "pairs_per_s": n * n / (ms / 1e3),
+             "roofline": {"bound": "mfma", "achieved": tflops, "peak": FP4_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": tflops / FP4_PEAK_TFLOPS,
+                          "int8_frac": tflops / INT8_PEAK_TOPS, "traffic": None,
+                          "output_hbm_frac": 4.0 * n * n / (ms / 1e3) / (HBM_PEAK_GBS * 1e9)}}
+        got = out.cpu().numpy()
+        if not a.no_cpu_baseline:
+            from oracle import oracle
+
+            cores = host_threads()
+            t0 = time.perf_counter()
+            want, _ = oracle.fingerprint_matrix(fa, fb, measure, nthreads=cores)
+            t = time.perf_counter() - t0
+            m["cpu_baseline"] = {"value": n * n / t, "unit": "pairs/s", "cores": cores,
+                                 "kind": "port",
+                                 "sample": f"all {n} x {n} pairs, {t:.1f} s, oracle/cs_oracle.c "
+                                           f"fingerprint_matrix ({measure}), float64, OpenMP"}
+            m["parity"] = {"pairs": int(got.size),
+                           "equal_to_float32_of_oracle": int((got == want.astype(np.float32)).sum()),
+                           "max_abs_err": float(np.abs(got.astype(np.float64) - want).max())}
+        log(f"fingerprint {measure}: {ms * 1e3:.1f} us/call, {m['roofline']['frac']:.3f} of fp4")
+        res["measures"][measure] = m
     return res
+
+
+def bench_fingerprint(a) -> dict:
+    """``--workload fingerprint``: the same section as its own bench line."""
+    fp = bench_fingerprint_measures(a)
+    j = fp["measures"]["jaccard"]
+    return {"metric": "FingerprintSimilarity jaccard pairs/s at 4096 x 4096 x 2048 bits",
+            "value": j["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+            "steps": max(a.steps, 10), "warmup": max(a.warmup, 1), "ms_per_step": j["ms_per_call"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp4",
+            "data": "synthetic", "config": fp["config"], "roofline": j["roofline"],
+            "cpu_baseline": j.get("cpu_baseline"), "fingerprint": fp}
+
+
+def bench_matrix_wall(a, packed) -> dict:
+    """What a user of the drop-in class waits for: CudaCosineGreedy.matrix on
+    lists of spectrum objects, wall clock (packing, upload, kernels, download
+    and the structured array), as the reference's own benchmark times it
+    (notebooks/figures/performance/benchmark.py:63-67)."""
+    import torch
+
+    from cudams.similarity import CudaCosineGreedy
+    from cudams.spectrum import Spectrum
+
+    off = packed.offsets
+    spectra = [Spectrum(packed.mz[off[i]:off[i + 1]], packed.intensities[off[i]:off[i + 1]])
+               for i in range(packed.n_spectra)]
+    cg = CudaCosineGreedy(tolerance=0.01, match_limit=2048)
+    cg.matrix(spectra[:64], spectra[:64])
+    times = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = cg.matrix(spectra, spectra)
+        times.append((time.perf_counter() - t0) * 1e3)
+        del out
+    best = min(times)
+    n = packed.n_spectra
+    log(f"matrix() wall {[round(t, 1) for t in times]} ms")
+    return {"what": "CudaCosineGreedy(tolerance=0.01).matrix(spectra, spectra) -> np.recarray",
+            "ms": best, "ms_runs": times, "pairs_per_s": n * n / (best / 1e3),
+            "kernel_ms": cg.kernel_time * 1e3}
 
 
 def bench_copy(a) -> dict:
@@ -344,7 +450,7 @@ def bench_serving(a) -> dict:
     if not a.no_cpu_baseline:
         from oracle import oracle
 
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        cores = host_threads()
         csr = (lib.mz, lib.intensities, lib.offsets)
         cpu = {}
         for size in (1, 16):
@@ -418,8 +524,13 @@ def main() -> int:
     ap.add_argument("--workload", choices=["cosine", "fingerprint", "copy"], default="cosine")
     ap.add_argument("--light-candidates", type=int, default=0)
     ap.add_argument("--round-capacity", type=int, default=0)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the comparison of the whole matrix with the oracle")
+    ap.add_argument("--no-matrix-wall", action="store_true",
+                    help="skip the end-to-end timing of the drop-in matrix() call")
+    ap.add_argument("--no-fingerprint", action="store_true",
+                    help="skip the fingerprint section of the cosine line")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-jobs section of the cosine line")
     ap.add_argument("--no-serving", action="store_true",
@@ -438,6 +549,8 @@ def main() -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
     res = {"cosine": bench_cosine, "fingerprint": bench_fingerprint, "copy": bench_copy}[a.workload](a)
+    if a.workload == "cosine" and not a.no_fingerprint:
+        res["fingerprint"] = bench_fingerprint_measures(a)
     if a.workload == "cosine" and not a.no_batched:
         res["batched_jobs"] = bench_batched(a)
     if a.workload == "cosine" and not a.no_serving:

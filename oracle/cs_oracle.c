@@ -35,6 +35,10 @@
  *   cso_fingerprint_matrix : the closed-form values of
  *       tests/similarity/test_fingerprint_similarity.py:8-112 and a brute-force
  *       popcount check.
+ *   cso_near_tie_pairs : constructed pairs whose two conflicting candidates
+ *       differ below float32 resolution (flagged) or above it (not flagged),
+ *       and a brute-force pure-Python restatement with float32 keys on tiny
+ *       random inputs.
  * The norm sum runs left to right; numpy's pairwise summation can differ in
  * the last bit of the norm (1e-16 relative), far below the 1e-5 score bar.
  */
@@ -211,6 +215,114 @@ int cso_cosine_greedy_matrix(const double *rmz, const double *rin, const int64_t
         free(used);
     }
     return 0;
+}
+
+/* Greedy acceptance over candidates sorted ascending (stable), walked in
+ * reverse: flags[k] = 1 iff sorted candidate k is accepted. */
+static int32_t greedy_flags(const cso_pair *a, int64_t c, unsigned char *u1, unsigned char *u2,
+                            unsigned char *flags) {
+    int32_t m = 0;
+    for (int64_t k = c - 1; k >= 0; --k) {
+        flags[k] = 0;
+        if (!u1[a[k].i] && !u2[a[k].j]) {
+            u1[a[k].i] = 1;
+            u2[a[k].j] = 1;
+            flags[k] = 1;
+            ++m;
+        }
+    }
+    return m;
+}
+
+static int cmp_ij(const void *x, const void *y) {
+    const cso_pair *a = (const cso_pair *)x, *b = (const cso_pair *)y;
+    if (a->i != b->i) return a->i < b->i ? -1 : 1;
+    if (a->j != b->j) return a->j < b->j ? -1 : 1;
+    return 0;
+}
+
+/* Near-tie exposure of every pair: near_tie[r*Q + q] = 1 iff ordering the
+ * candidates by their values rounded to float32 (as the reference GPU kernel
+ * stores them, spectrum_similarity_functions.py:117,147; same stable sort and
+ * reversal, i.e. ties to the later-collected candidate) makes the greedy accept
+ * a different set of peak pairs than the float64 order of matchms.  These are
+ * the pairs whose assignment hinges on values closer than float32 can tell
+ * apart.  Returns the number of such pairs. */
+int64_t cso_near_tie_pairs(const double *rmz, const double *rin, const int64_t *roff, int64_t R,
+                           const double *qmz, const double *qin, const int64_t *qoff, int64_t Q,
+                           double tol, double shift, double mz_power, double int_power,
+                           uint8_t *near_tie, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    int64_t total = 0;
+#pragma omp parallel reduction(+ : total)
+    {
+        cso_pair *a = NULL, *b = NULL, *tmp = NULL;
+        unsigned char *used = NULL, *fa = NULL, *fb = NULL;
+        int64_t cap = 0, ucap = 0;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t r = 0; r < R; ++r) {
+            const int64_t n1 = roff[r + 1] - roff[r];
+            for (int64_t q = 0; q < Q; ++q) {
+                const int64_t n2 = qoff[q + 1] - qoff[q];
+                const double *m1 = rmz + roff[r], *i1 = rin + roff[r];
+                const double *m2 = qmz + qoff[q], *i2 = qin + qoff[q];
+                int64_t c = collect_peak_pairs(m1, i1, n1, m2, i2, n2, tol, shift, mz_power,
+                                               int_power, NULL, 0);
+                uint8_t flag = 0;
+                if (c >= 2) {
+                    if (c > cap) {
+                        cap = c;
+                        a = (cso_pair *)realloc(a, (size_t)c * sizeof(cso_pair));
+                        b = (cso_pair *)realloc(b, (size_t)c * sizeof(cso_pair));
+                        tmp = (cso_pair *)realloc(tmp, (size_t)c * sizeof(cso_pair));
+                        fa = (unsigned char *)realloc(fa, (size_t)c);
+                        fb = (unsigned char *)realloc(fb, (size_t)c);
+                    }
+                    if (n1 + n2 > ucap) {
+                        ucap = n1 + n2;
+                        used = (unsigned char *)realloc(used, (size_t)ucap);
+                    }
+                    collect_peak_pairs(m1, i1, n1, m2, i2, n2, tol, shift, mz_power, int_power,
+                                       a, c);
+                    memcpy(b, a, (size_t)c * sizeof(cso_pair));
+                    for (int64_t k = 0; k < c; ++k) b[k].v = (double)(float)b[k].v;
+                    stable_sort_ascending(a, tmp, c);
+                    stable_sort_ascending(b, tmp, c);
+                    memset(used, 0, (size_t)(n1 + n2));
+                    greedy_flags(a, c, used, used + n1, fa);
+                    memset(used, 0, (size_t)(n1 + n2));
+                    greedy_flags(b, c, used, used + n1, fb);
+                    /* compare the accepted sets: keep accepted (i, j), sorted */
+                    int64_t na = 0, nb = 0;
+                    for (int64_t k = 0; k < c; ++k)
+                        if (fa[k]) a[na++] = a[k];
+                    for (int64_t k = 0; k < c; ++k)
+                        if (fb[k]) b[nb++] = b[k];
+                    if (na != nb) {
+                        flag = 1;
+                    } else {
+                        qsort(a, (size_t)na, sizeof(cso_pair), cmp_ij);
+                        qsort(b, (size_t)nb, sizeof(cso_pair), cmp_ij);
+                        for (int64_t k = 0; k < na && !flag; ++k)
+                            flag = (a[k].i != b[k].i) || (a[k].j != b[k].j);
+                    }
+                }
+                if (near_tie) near_tie[r * Q + q] = flag;
+                total += flag;
+            }
+        }
+        free(a);
+        free(b);
+        free(tmp);
+        free(used);
+        free(fa);
+        free(fb);
+    }
+    return total;
 }
 
 /* Fingerprint measures on 0/1 byte fingerprints [rows, nbits], float64.

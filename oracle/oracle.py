@@ -51,6 +51,9 @@ def lib():
                                                     _p, _p, _i64, _p, _p]
         L.cso_cosine_greedy_matrix.argtypes = [_p, _p, _p, _i64, _p, _p, _p, _i64, _d, _d, _d, _d,
                                                _i64, _p, _p, _p, _p, ctypes.c_int]
+        L.cso_near_tie_pairs.argtypes = [_p, _p, _p, _i64, _p, _p, _p, _i64, _d, _d, _d, _d, _p,
+                                         ctypes.c_int]
+        L.cso_near_tie_pairs.restype = _i64
         L.cso_fingerprint_matrix.argtypes = [_p, _i64, _p, _i64, _i64, ctypes.c_int, _p, _p,
                                              ctypes.c_int]
         _lib = L
@@ -129,6 +132,42 @@ def cosine_greedy_matrix(refs, queries, tolerance=0.1, shift=0.0, mz_power=0.0,
                                    int(match_limit), _ptr(score), _ptr(matches), _ptr(overflow),
                                    _ptr(ncand), int(nthreads))
     return {"score": score, "matches": matches, "overflow": overflow, "n_candidates": ncand}
+
+
+def near_tie_pairs(refs, queries, tolerance=0.1, shift=0.0, mz_power=0.0, intensity_power=1.0,
+                   nthreads=0):
+    """uint8 [R, Q]: 1 where a greedy over candidate values rounded to float32
+    (the reference GPU kernel's stored products) accepts a different set of
+    peak pairs than the float64 order of matchms."""
+    rmz, rin, roff = refs if isinstance(refs, tuple) and len(refs) == 3 else pack(refs)
+    qmz, qin, qoff = queries if isinstance(queries, tuple) and len(queries) == 3 else pack(queries)
+    rmz, rin, qmz, qin = _f64(rmz), _f64(rin), _f64(qmz), _f64(qin)
+    roff = np.ascontiguousarray(roff, np.int64)
+    qoff = np.ascontiguousarray(qoff, np.int64)
+    R, Q = len(roff) - 1, len(qoff) - 1
+    flags = np.zeros((R, Q), np.uint8)
+    lib().cso_near_tie_pairs(_ptr(rmz), _ptr(rin), _ptr(roff), R, _ptr(qmz), _ptr(qin),
+                             _ptr(qoff), Q, tolerance, shift, mz_power, intensity_power,
+                             _ptr(flags), int(nthreads))
+    return flags
+
+
+def greedy_accepted_py(mz1, int1, mz2, int2, tolerance=0.1, key=float):
+    """Pure-Python accepted set {(i, j)} of the greedy with candidate values
+    mapped by ``key`` before ordering (v desc, i desc, j desc)."""
+    cands = []
+    for i in range(len(mz1)):
+        for j in range(len(mz2)):
+            if mz1[i] - tolerance <= mz2[j] <= mz1[i] + tolerance:
+                cands.append((key(float(int1[i]) * float(int2[j])), i, j))
+    cands.sort(reverse=True)
+    u1, u2, acc = set(), set(), set()
+    for _, i, j in cands:
+        if i not in u1 and j not in u2:
+            u1.add(i)
+            u2.add(j)
+            acc.add((i, j))
+    return acc
 
 
 MEASURES = {"jaccard": 0, "dice": 1, "cosine": 2}

@@ -229,3 +229,37 @@ def test_fingerprint_counts_brute_force(oracle):
     assert s[2, 3] == pytest.approx(2 * np.sum(a[2] & b[3]) / (a[2].sum() + b[3].sum()))
     s, _ = oracle.fingerprint_matrix(a, b, "cosine")
     assert s[4, 1] == pytest.approx(np.sum(a[4] & b[1]) / np.sqrt(a[4].sum() * b[1].sum()))
+
+
+def test_near_tie_pairs_constructed(oracle):
+    """Two candidates share query peak 0.  Values 0.3 + 1e-12 and 0.3 differ in
+    float64 but round to the same float32: the float64 order accepts the first
+    collected, the float32 order breaks the tie towards the later one."""
+    q = ([100.0], [1.0])
+    near = ([100.0, 100.004], [0.3 + 1e-12, 0.3])
+    far = ([100.0, 100.004], [0.4, 0.3])
+    later_larger = ([100.0, 100.004], [0.3, 0.3 + 1e-12])
+    flags = oracle.near_tie_pairs([near, far, later_larger], [q], tolerance=0.01)
+    assert flags[:, 0].tolist() == [1, 0, 0]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_near_tie_pairs_brute_force(oracle, seed):
+    """Against a pure-Python greedy run once with float64 keys and once with
+    keys rounded to float32, on tiny inputs whose intensities are drawn from a
+    few values a few float32 ulps apart (many near ties)."""
+    rng = np.random.default_rng(100 + seed)
+    base = np.array([0.3, 0.3 + 1e-12, 0.3 + 3e-12, 0.7, 0.7 - 2e-13])
+    spectra = []
+    for _ in range(10):
+        n = int(rng.integers(1, 6))
+        mz = np.sort(rng.choice(np.arange(100.0, 102.0, 0.25), n, replace=False))
+        spectra.append((mz, rng.choice(base, n)))
+    flags = oracle.near_tie_pairs(spectra, spectra, tolerance=0.3)
+    f32 = lambda v: float(np.float32(v))  # noqa: E731
+    for r, (m1, i1) in enumerate(spectra):
+        for c, (m2, i2) in enumerate(spectra):
+            a = oracle.greedy_accepted_py(m1, i1, m2, i2, 0.3)
+            b = oracle.greedy_accepted_py(m1, i1, m2, i2, 0.3, key=f32)
+            assert flags[r, c] == int(a != b), (r, c)
+    assert flags.sum() > 0
