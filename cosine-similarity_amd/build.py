@@ -1,11 +1,13 @@
-"""Compile the gfx950 HIP sources of this package into cudams/libcsamd.so.
+"""Compile the native libraries of this package.
 
 Usage: python cosine-similarity_amd/build.py [--jobs N]
 
-Every .hip file under csrc/ is compiled for gfx950 with hipcc (one object per
-file, in parallel) and linked into one shared library with a plain C ABI
-(include/cosine_similarity_amd.h).  hipcc cross-compiles, so this runs on a
-machine without a GPU.
+* cudams/libcsamd.so: every .hip file under csrc/, compiled for gfx950 with
+  hipcc (one object per file, in parallel) and linked into one shared library
+  with a plain C ABI (include/cosine_similarity_amd.h).  hipcc cross-compiles,
+  so this runs on a machine without a GPU.
+* cudams/libcsamd_cpu.so: the host engine under csrc/cpu/ (g++, OpenMP), the
+  compute of CPUParallelCosineGreedy; it needs no GPU.
 """
 from __future__ import annotations
 
@@ -22,6 +24,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "cudams", "libcsamd.so")
+CPU_LIB = os.path.join(HERE, "cudams", "libcsamd_cpu.so")
 ARCH = "gfx950"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -56,6 +59,27 @@ def _compile(src: str) -> str:
     return obj
 
 
+# host engine: float64 products and sums must round as matchms writes them
+CPU_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wextra",
+             "-Wno-unknown-pragmas", f"-I{INCLUDE}", f"-I{CSRC}"]
+
+
+def cpu_sources():
+    d = os.path.join(CSRC, "cpu")
+    return sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".cpp"))
+
+
+def build_cpu(out: str = CPU_LIB, extra=()) -> str:
+    """g++ build of the host engine into ``out`` (``extra``: more flags)."""
+    tmp = out + ".tmp"
+    cmd = ["g++", *CPU_FLAGS, *extra, "-shared", "-o", tmp, *cpu_sources()]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, out)
+    return out
+
+
 def build(jobs: int = 4, verbose: bool = True) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
     srcs = sources()
@@ -67,8 +91,9 @@ def build(jobs: int = 4, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cpu()
     if verbose:
-        print(f"built {LIB} from {len(srcs)} sources", flush=True)
+        print(f"built {LIB} from {len(srcs)} sources and {CPU_LIB}", flush=True)
     return LIB
 
 

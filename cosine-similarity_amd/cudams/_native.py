@@ -12,6 +12,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libcsamd.so")
+CPU_LIB_PATH = os.path.join(_HERE, "libcsamd_cpu.so")
 
 CS_OK = 0
 CS_ERR_INVALID = 1
@@ -166,6 +167,39 @@ def load(path: str = LIB_PATH):
                                        p, p, p, p, p, p]
     _lib = L
     return L
+
+
+# Every symbol of the host library (libcsamd_cpu.so).
+CPU_EXPORTED_SYMBOLS = ("cs_cpu_last_error", "cs_cpu_cosine_greedy")
+
+_cpu_lib = None
+
+
+def load_cpu(path: str = CPU_LIB_PATH):
+    """Load libcsamd_cpu.so (the host engine of CPUParallelCosineGreedy)."""
+    global _cpu_lib
+    if _cpu_lib is not None:
+        return _cpu_lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run `python cosine-similarity_amd/build.py`.")
+    L = ctypes.CDLL(path)
+    p = ctypes.c_void_p
+    L.cs_cpu_last_error.restype = ctypes.c_char_p
+    L.cs_cpu_last_error.argtypes = []
+    L.cs_cpu_cosine_greedy.restype = ctypes.c_int
+    L.cs_cpu_cosine_greedy.argtypes = [ctypes.POINTER(CsSpectra), ctypes.POINTER(CsSpectra),
+                                       ctypes.c_double, ctypes.c_double, p, p, ctypes.c_int64,
+                                       ctypes.c_int]
+    _cpu_lib = L
+    return L
+
+
+def check_cpu(status: int) -> None:
+    if status != CS_OK:
+        msg = load_cpu().cs_cpu_last_error().decode(errors="replace")
+        if status == CS_ERR_INVALID:
+            raise ValueError(f"libcsamd_cpu: {msg}")
+        raise NativeError(status, msg)
 
 
 def check(status: int) -> None:

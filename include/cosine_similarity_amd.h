@@ -282,6 +282,31 @@ cs_status cs_threshold_compact(const float *score, const int32_t *matches,
                                int64_t *out_col, float *out_score, int32_t *out_matches,
                                uint8_t *out_overflow, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Host engine (libcsamd_cpu.so: g++ + OpenMP, no GPU needed)                 */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failing cs_cpu_* call on this host thread. */
+const char *cs_cpu_last_error(void);
+
+/*
+ * CosineGreedy for every (reference, query) pair on the host cores:
+ * score[r*ld + q] (float32 of the float64 greedy score) and matches[r*ld + q],
+ * matchms CosineGreedy.pair semantics as cs_cosine_greedy, without the
+ * overflow plane (CPUParallelCosineGreedy.score_datatype has none).  Every
+ * pointer in `refs` / `queries` is HOST memory; the whole CSR is validated
+ * first (offsets non-decreasing and within n_peaks, lengths <= max_peaks,
+ * m/z finite, ascending per spectrum and within max_abs_mz): a violation
+ * returns CS_ERR_INVALID.  n_threads <= 0 uses OpenMP's default.
+ * Replaces the numba prange kernel cpu_parallel_cosine_greedy_kernel
+ * (cudams/similarity/spectrum_similarity_functions.py:281-320) called from
+ * CPUParallelCosineGreedy._matrix_with_numpy_output
+ * (cudams/similarity/CPUParallelCosineGreedy.py:50-70).
+ */
+cs_status cs_cpu_cosine_greedy(const cs_spectra *refs, const cs_spectra *queries,
+                               double tolerance, double shift, float *score, int32_t *matches,
+                               int64_t ld, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

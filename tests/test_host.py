@@ -19,12 +19,15 @@ def test_library_exports_every_declared_symbol():
     from cudams import _native
 
     L = _native.load()
+    C = _native.load_cpu()
     with open(os.path.join(REPO, "include", "cosine_similarity_amd.h")) as fh:
         hdr = fh.read()
     declared = set(re.findall(r"\b(cs_[a-z0-9_]+)\s*\(", hdr))
-    assert declared == set(_native.EXPORTED_SYMBOLS)
-    for name in declared:
+    assert declared == set(_native.EXPORTED_SYMBOLS) | set(_native.CPU_EXPORTED_SYMBOLS)
+    for name in _native.EXPORTED_SYMBOLS:
         assert hasattr(L, name), name
+    for name in _native.CPU_EXPORTED_SYMBOLS:
+        assert hasattr(C, name), name
     assert L.cs_version() >= 10000
     assert L.cs_last_error() == b""
 
