@@ -525,11 +525,17 @@ constexpr size_t kWaveLds =
     ((size_t)kPool * (8 + 4 + 1) + (size_t)kTileQueries * (4 + 1 + kIdxRanks + 1) + 16 + kWave * 4 +
      15) / 16 * 16;
 constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
+// Slots of the block-relative reference offsets in LDS: kMaxLightRefs + 1
+// rounded up so that the pools after them start 16-byte aligned (their
+// float64 values are read and written as 8-byte LDS accesses).
+constexpr int kRoffSlots = (kMaxLightRefs + 1 + 3) / 4 * 4;
+constexpr size_t kPoolBase = (size_t)kTileQueries * (8 + 8) + (size_t)kRoffSlots * 4;
+static_assert(kPoolBase % 16 == 0 && kWaveLds % 16 == 0, "pool values are 8-byte aligned");
 
 // Bytes of dynamic LDS of k_light for a tile capacity `cap`.
 constexpr size_t light_lds_bytes(int cap) {
     return (size_t)kTileQueries * (8 + 8)          // qoff, qnorm
-           + (size_t)(kMaxLightRefs + 1) * 4        // reference offsets (block-relative)
+           + (size_t)kRoffSlots * 4                 // reference offsets (block-relative)
            + (size_t)kLightWaves * kWaveLds         // pools
            + (size_t)(cap + 1) * 4                  // sorted values (f32)
            + (size_t)(cap + 1) * 4                  // sorted int_pow (f32)
@@ -639,7 +645,7 @@ __device__ __forceinline__ WavePool wave_pool(unsigned char *lds, int wave) {
     int64_t *qoff_l = reinterpret_cast<int64_t *>(lds);
     double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
     int32_t *roff_l = reinterpret_cast<int32_t *>(qnorm_l + kTileQueries);
-    unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kMaxLightRefs + 1);
+    unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kRoffSlots);
     WavePool w;
     w.pv = reinterpret_cast<double *>(pools + wave * kWaveLds);
     w.pij = reinterpret_cast<uint32_t *>(w.pv + kPool);
@@ -722,7 +728,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
     double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
     int32_t *roff_l = reinterpret_cast<int32_t *>(qnorm_l + kTileQueries);
-    unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kMaxLightRefs + 1);
+    unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kRoffSlots);
     unsigned char *wbase = pools + wave * kWaveLds;
     double *pv = reinterpret_cast<double *>(wbase);
     uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
@@ -2306,6 +2312,22 @@ cs_status check_call(const cs_spectra *refs, const cs_spectra *queries,
         return fail(CS_ERR_INVALID, "round_capacity must be a power of two in [4, 1024]");
     if (params->refs_per_block < 0 || params->refs_per_block > kMaxLightRefs)
         return fail(CS_ERR_INVALID, "refs_per_block must be in [0, %d]", kMaxLightRefs);
+    if (computes) {
+        // the cooperative kernels' LDS, checked before anything is launched:
+        // a light stage without its heavy stage would leave -1 sentinels
+        const int r_words = (refs->max_peaks + 31) / 32 + 1;
+        const int q_words = (queries->max_peaks + 31) / 32 + 1;
+        const int stage = (refs->max_peaks <= kHeavyStage && queries->max_peaks <= kHeavyStage);
+        int cap = 1;
+        while (cap < 2 * T + kWave) cap <<= 1;
+        const size_t hlds = heavy_lds_bytes(r_words, q_words, stage);
+        const size_t glds = giant_lds_bytes(cap, r_words, q_words);
+        if (hlds > 160 * 1024 || glds > 65536)
+            return fail(CS_ERR_UNSUPPORTED,
+                        "max_peaks %d/%d with round_capacity %d need %zu / %zu bytes of LDS "
+                        "(limits 160 KiB / 64 KiB)",
+                        refs->max_peaks, queries->max_peaks, T, hlds, glds);
+    }
     return CS_OK;
 }
 
@@ -2453,6 +2475,15 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
         b->npeaks += J.queries.n_peaks;
         const int64_t nr = J.refs.n_spectra;
         b->nrb.push_back(nr && nt ? (nr + b->rpb - 1) / b->rpb : 0);
+        if (b->nrb.back() >= (int64_t(1) << 24)) {
+            // a work item carries the reference block in 24 bits
+            const long long blocks = (long long)b->nrb.back();
+            const int rpb = b->rpb;
+            delete b;
+            return fail(CS_ERR_UNSUPPORTED,
+                        "job %d: %lld reference blocks of %d references (at most 2^24 - 1 per "
+                        "job; raise refs_per_block)", j, blocks, rpb);
+        }
     }
     if (b->ntiles >= (int64_t(1) << 24) * 1) {
         // tile indices of one job travel in 24 bits; the sum bounds them all

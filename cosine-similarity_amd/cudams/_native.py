@@ -39,7 +39,14 @@ EXPORTED_SYMBOLS = (
     "cs_topk_rows_workspace_size",
     "cs_topk_rows",
     "cs_threshold_compact",
+    "cs_validate_spectra",
 )
+
+CS_FAULT_OFFSETS, CS_FAULT_LENGTH, CS_FAULT_MZ_RANGE, CS_FAULT_MZ_ORDER = 1, 2, 4, 8
+FAULT_NAMES = {CS_FAULT_OFFSETS: "offsets decrease, are negative or pass n_peaks",
+               CS_FAULT_LENGTH: "a spectrum has more peaks than max_peaks",
+               CS_FAULT_MZ_RANGE: "an m/z is not finite or exceeds max_abs_mz",
+               CS_FAULT_MZ_ORDER: "m/z not ascending within a spectrum"}
 
 
 class CsSpectra(ctypes.Structure):
@@ -165,6 +172,8 @@ def load(path: str = LIB_PATH):
     L.cs_threshold_compact.restype = i32
     L.cs_threshold_compact.argtypes = [p, p, p, i64, i64, i64, ctypes.c_float, i64, p, p, i64,
                                        p, p, p, p, p, p]
+    L.cs_validate_spectra.restype = i32
+    L.cs_validate_spectra.argtypes = [ctypes.POINTER(CsSpectra), p, p]
     _lib = L
     return L
 

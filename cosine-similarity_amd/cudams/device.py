@@ -85,6 +85,26 @@ def cosine_greedy(refs: DeviceSpectra, queries: DeviceSpectra, tolerance: float,
                                      _stream_ptr(stream)))
 
 
+def validate_spectra(spectra, stream=None) -> int:
+    """CS_FAULT_* bits of a device-resident spectrum set (0 = sound), from the
+    device validator cs_validate_spectra; synchronises ``stream``."""
+    require_gpu()
+    L = _native.load()
+    s = spectra.struct()
+    dev = spectra.offsets.device if hasattr(spectra, "offsets") else spectra.parent.offsets.device
+    flag = torch.empty(1, dtype=torch.int32, device=dev)
+    _native.check(L.cs_validate_spectra(ctypes.byref(s), flag.data_ptr(), _stream_ptr(stream)))
+    return int(flag.item()) & 0xffffffff
+
+
+def check_spectra_on_device(spectra, stream=None) -> None:
+    """Raises ValueError naming every fault the device validator finds."""
+    f = validate_spectra(spectra, stream)
+    if f:
+        raise ValueError("corrupt spectrum set: " + "; ".join(
+            msg for bit, msg in _native.FAULT_NAMES.items() if f & bit))
+
+
 MEASURES = {"jaccard": _native.CS_JACCARD, "dice": _native.CS_DICE, "cosine": _native.CS_COSINE}
 
 

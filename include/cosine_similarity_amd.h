@@ -73,6 +73,33 @@ typedef struct cs_spectra {
 } cs_spectra;
 
 /*
+ * Fault bits of cs_validate_spectra / cs_cpu_cosine_greedy's input check.
+ */
+enum {
+    CS_FAULT_OFFSETS = 1,   /* offsets[0] < 0, decreasing, or past offsets[0] + n_peaks */
+    CS_FAULT_LENGTH = 2,    /* a spectrum longer than max_peaks                         */
+    CS_FAULT_MZ_RANGE = 4,  /* an m/z not finite or above max_abs_mz in magnitude       */
+    CS_FAULT_MZ_ORDER = 8   /* m/z not ascending within a spectrum                      */
+};
+
+/*
+ * Checks a device-resident spectrum set before it is trusted:
+ * *fault (a device uint32) receives the OR of the CS_FAULT_* bits over every
+ * spectrum, 0 when the set is sound.  Asynchronous on `stream` (the flag is
+ * cleared by a kernel first, so the call can sit in a captured graph); read
+ * the flag after the stream is synchronised.  Peak arrays are read only
+ * within [offsets[0], offsets[0] + n_peaks), and only for spectra whose own
+ * offsets lie inside it.  The entry points that compute take max_peaks,
+ * max_abs_mz, the offsets and the m/z order on trust (an understated
+ * max_abs_mz narrows the probe window, unsorted m/z breaks the tile merge):
+ * run this on any set the caller did not pack itself.  Replaces the checks
+ * the reference leaves to matchms's Spectrum constructor (sorted float64
+ * m/z) and to CudaCosineGreedy._spectra_peaks_to_tensor's truncation
+ * (CudaCosineGreedy.py:63-87).
+ */
+cs_status cs_validate_spectra(const cs_spectra *spectra, uint32_t *fault, void *stream);
+
+/*
  * CosineGreedy parameters (CudaCosineGreedy.py:24-56).  mz_power and
  * intensity_power are folded into cs_spectra.mz_pow / int_pow.
  */

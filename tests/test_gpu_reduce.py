@@ -78,3 +78,52 @@ def test_threshold_compact_matches_numpy(gpu):
     np.testing.assert_array_equal(sc, s[rr, cc])
     np.testing.assert_array_equal(mm, m[rr, cc])
     np.testing.assert_array_equal(oo, o[rr, cc])
+
+
+@pytest.mark.parametrize("bad", ["none", "unsorted", "offsets_past_end", "offsets_decrease",
+                                 "max_peaks", "max_abs_mz", "nan", "exact_unsorted"])
+def test_device_validator_flags_each_fault(gpu, bad):
+    """cs_validate_spectra over a set whose device arrays were corrupted after
+    packing: each kind of corruption raises its own fault bit, and only that
+    one (the compute entry points take these properties on trust)."""
+    import torch
+
+    from cudams import _native
+    from cudams.device import check_spectra_on_device, validate_spectra
+    from cudams.spectra import DeviceSpectra, PackedSpectra
+
+    mzs = [[100.0, 101.0, 102.0], [50.0, 60.0], [], [70.0, 70.5, 71.0, 80.0]]
+    if bad == "exact_unsorted":
+        mzs[0] = [100.0, 100.1, 102.0]  # not float32 numbers: mz_exact is kept
+    p = PackedSpectra.from_arrays(mzs, [[1.0] * len(m) for m in mzs])
+    S = DeviceSpectra(p, 0.0, 1.0, "cuda")
+    want = {"none": 0, "unsorted": _native.CS_FAULT_MZ_ORDER,
+            "offsets_past_end": _native.CS_FAULT_OFFSETS,
+            # [0, 3, 5, 4, 9]: spectrum 2 runs backwards, spectrum 3 spans 5 peaks
+            "offsets_decrease": _native.CS_FAULT_OFFSETS | _native.CS_FAULT_LENGTH,
+            "max_peaks": _native.CS_FAULT_LENGTH,
+            "max_abs_mz": _native.CS_FAULT_MZ_RANGE, "nan": _native.CS_FAULT_MZ_RANGE,
+            "exact_unsorted": _native.CS_FAULT_MZ_ORDER}[bad]
+    if bad == "unsorted":
+        S.mz[5] = 75.0
+    elif bad == "offsets_past_end":
+        S.offsets[4] = 10 ** 12
+    elif bad == "offsets_decrease":
+        S.offsets[3] = 4
+    elif bad == "max_peaks":
+        S.max_peaks = 3
+    elif bad == "max_abs_mz":
+        S.max_abs_mz = 90.0
+    elif bad == "nan":
+        S.mz[3] = float("nan")
+    elif bad == "exact_unsorted":
+        assert S.mz_exact is not None
+        S.mz_exact[1] = 99.0
+    torch.cuda.synchronize()
+    print(f"validator case {bad}", flush=True)
+    assert validate_spectra(S) == want
+    if want:
+        with pytest.raises(ValueError, match="corrupt spectrum set"):
+            check_spectra_on_device(S)
+    else:
+        check_spectra_on_device(S)

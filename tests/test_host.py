@@ -210,3 +210,46 @@ def test_workspace_reuse_never_reallocates():
     assert w.get(500, keep=True) is b
     with pytest.raises(ValueError):
         w.get(5000, keep=True)
+
+
+def _dummy_spectra(n, max_peaks):
+    from cudams import _native
+
+    fake = 0x1000  # never dereferenced: these calls fail in their host checks
+    return _native.CsSpectra(n, n, fake, fake, None, None, fake, fake, max_peaks, 1000.0)
+
+
+def test_lds_limits_rejected_before_any_launch():
+    """Spectra of 65535 peaks with round_capacity 1024 need more LDS than the
+    giant-pair kernel has: the call fails in its argument checks, before a
+    kernel is queued (so no output is left half written)."""
+    from cudams import _native
+
+    L = _native.load()
+    big, q = _dummy_spectra(4, 65535), _dummy_spectra(4, 65535)
+    p = _native.CsCosineParams(0.1, 0.0, 2048, 0, 1024, 0, 0)
+    st = L.cs_cosine_greedy(ctypes.byref(big), ctypes.byref(q), ctypes.byref(p), 0x1000, 0x1000,
+                            0x1000, 4, None, 0, None)
+    assert st == _native.CS_ERR_UNSUPPORTED
+    assert b"LDS" in L.cs_last_error()
+
+
+def test_batch_rejects_reference_blocks_beyond_24_bits():
+    """A work item carries its reference block in 24 bits: a job that would
+    need 2^24 or more blocks is refused at planning time."""
+    from cudams import _native
+
+    L = _native.load()
+    job = _native.CsCosineJob()
+    job.refs = _dummy_spectra((1 << 24) + 1, 8)
+    job.queries = _dummy_spectra(1, 8)
+    job.tolerance, job.shift, job.match_limit = 0.1, 0.0, 2048
+    job.score = job.matches = job.overflow = 0x1000
+    job.ld = 1
+    bp = _native.CsBatchParams(0, 0, 1, 0)
+    handle, nbytes = ctypes.c_void_p(), ctypes.c_size_t()
+    st = L.cs_cosine_batch_create(ctypes.byref(job), 1, ctypes.byref(bp), ctypes.byref(handle),
+                                  ctypes.byref(nbytes))
+    assert st == _native.CS_ERR_UNSUPPORTED
+    assert b"reference blocks" in L.cs_last_error()
+    assert not handle.value
