@@ -24,6 +24,7 @@
 #include <limits.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -2361,6 +2362,73 @@ extern "C" cs_status cs_cosine_greedy(const cs_spectra *refs, const cs_spectra *
         return fail(CS_ERR_WORKSPACE, "workspace of %zu bytes < %zu required", workspace_bytes,
                     total);
     char *base = static_cast<char *>(workspace);
+    const Layout L = layout(base, base + need.index_bytes, *queries);
+    return cosine_greedy_run(refs, queries, params, score, matches, overflow, ld, L, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the workspace is a file-scope device array
+// ---------------------------------------------------------------------------
+
+// The workspace of cs_cosine_greedy_noalloc.  Every GPU the library runs on
+// holds its own copy from module load until the process ends.  In host code
+// the name is only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[CS_NOALLOC_WORKSPACE_BYTES];
+
+namespace cs {
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+struct NoAllocSlot {
+    std::once_flag once;
+    void *ptr = nullptr;
+    hipError_t err = hipSuccess;
+};
+NoAllocSlot g_noalloc[kMaxDevices];
+
+// Device address of `scratch` on the current device, looked up on the first
+// call on that device (with that device current) and kept.
+cs_status noalloc_workspace(void **out) {
+    int dev = -1;
+    CS_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(CS_ERR_UNSUPPORTED, "device ordinal %d beyond %d", dev, kMaxDevices);
+    NoAllocSlot &s = g_noalloc[dev];
+    std::call_once(s.once, [&s]() {
+        s.err = hipGetSymbolAddress(&s.ptr, HIP_SYMBOL(scratch));
+        if (s.err == hipSuccess && s.ptr == nullptr) s.err = hipErrorInvalidSymbol;
+    });
+    if (s.err != hipSuccess)
+        return fail(CS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                    hipGetErrorString(s.err));
+    *out = s.ptr;
+    return CS_OK;
+}
+
+}  // namespace
+}  // namespace cs
+
+extern "C" cs_status cs_cosine_greedy_noalloc(const cs_spectra *refs, const cs_spectra *queries,
+                                              const cs_cosine_params *params, float *score,
+                                              int32_t *matches, uint8_t *overflow, int64_t ld,
+                                              void *stream) {
+    cs_status st;
+    if ((st = check_call(refs, queries, params, score, matches, overflow, ld)) != CS_OK) return st;
+    if (queries->n_spectra == 0) return CS_OK;
+    if (refs->n_spectra == 0 && !(params->stages & CS_STAGE_PREPARE)) return CS_OK;
+    const Layout need = layout(nullptr, nullptr, *queries);
+    const size_t total = need.index_bytes + need.scratch_bytes;
+    if (total > (size_t)CS_NOALLOC_WORKSPACE_BYTES)
+        return fail(CS_ERR_UNSUPPORTED,
+                    "%lld queries of %lld peaks need %zu bytes of workspace; the allocation-free "
+                    "entry point holds %zu (use cs_cosine_greedy with a workspace)",
+                    (long long)queries->n_spectra, (long long)queries->n_peaks, total,
+                    (size_t)CS_NOALLOC_WORKSPACE_BYTES);
+    void *ws = nullptr;
+    if ((st = noalloc_workspace(&ws)) != CS_OK) return st;
+    char *base = static_cast<char *>(ws);
     const Layout L = layout(base, base + need.index_bytes, *queries);
     return cosine_greedy_run(refs, queries, params, score, matches, overflow, ld, L, stream);
 }

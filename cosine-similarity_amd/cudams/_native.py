@@ -40,7 +40,10 @@ EXPORTED_SYMBOLS = (
     "cs_topk_rows",
     "cs_threshold_compact",
     "cs_validate_spectra",
+    "cs_cosine_greedy_noalloc",
 )
+
+NOALLOC_WORKSPACE_BYTES = 64 << 20  # CS_NOALLOC_WORKSPACE_BYTES
 
 CS_FAULT_OFFSETS, CS_FAULT_LENGTH, CS_FAULT_MZ_RANGE, CS_FAULT_MZ_ORDER = 1, 2, 4, 8
 FAULT_NAMES = {CS_FAULT_OFFSETS: "offsets decrease, are negative or pass n_peaks",
@@ -172,6 +175,9 @@ def load(path: str = LIB_PATH):
     L.cs_threshold_compact.restype = i32
     L.cs_threshold_compact.argtypes = [p, p, p, i64, i64, i64, ctypes.c_float, i64, p, p, i64,
                                        p, p, p, p, p, p]
+    L.cs_cosine_greedy_noalloc.restype = i32
+    L.cs_cosine_greedy_noalloc.argtypes = [ctypes.POINTER(CsSpectra), ctypes.POINTER(CsSpectra),
+                                           ctypes.POINTER(CsCosineParams), p, p, p, i64, p]
     L.cs_validate_spectra.restype = i32
     L.cs_validate_spectra.argtypes = [ctypes.POINTER(CsSpectra), p, p]
     _lib = L

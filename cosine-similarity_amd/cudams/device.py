@@ -85,6 +85,30 @@ def cosine_greedy(refs: DeviceSpectra, queries: DeviceSpectra, tolerance: float,
                                      _stream_ptr(stream)))
 
 
+def cosine_greedy_noalloc(refs, queries, tolerance: float, shift: float, match_limit: int,
+                          score: torch.Tensor, matches: torch.Tensor, overflow: torch.Tensor,
+                          light_candidates: int = 0, round_capacity: int = 0, stream=None,
+                          stages: int = 0) -> None:
+    """cs_cosine_greedy_noalloc: as :func:`cosine_greedy`, with the library's
+    own fixed device workspace (nothing is allocated; overlapping calls on one
+    GPU race)."""
+    require_gpu()
+    L = _native.load()
+    R, Q = refs.n_spectra, queries.n_spectra
+    for t, dt in ((score, torch.float32), (matches, torch.int32), (overflow, torch.uint8)):
+        if t.dtype != dt or not t.is_cuda or t.dim() != 2 or t.shape[0] < R or t.shape[1] < Q:
+            raise ValueError(f"output must be a CUDA {dt} tensor of shape >= ({R}, {Q})")
+        if t.stride(1) != 1 or t.stride(0) != score.stride(0):
+            raise ValueError("outputs must be row-major with one common row stride")
+    rs, qs = refs.struct(), queries.struct()
+    p = _native.CsCosineParams(float(tolerance), float(shift), int(match_limit),
+                               int(light_candidates), int(round_capacity), 0, int(stages))
+    _native.check(L.cs_cosine_greedy_noalloc(ctypes.byref(rs), ctypes.byref(qs), ctypes.byref(p),
+                                             score.data_ptr(), matches.data_ptr(),
+                                             overflow.data_ptr(), score.stride(0),
+                                             _stream_ptr(stream)))
+
+
 def validate_spectra(spectra, stream=None) -> int:
     """CS_FAULT_* bits of a device-resident spectrum set (0 = sound), from the
     device validator cs_validate_spectra; synchronises ``stream``."""

@@ -181,6 +181,28 @@ cs_status cs_cosine_greedy_indexed(const cs_spectra *refs, const cs_spectra *que
                                    uint8_t *overflow, int64_t ld, void *index, size_t index_bytes,
                                    void *scratch, size_t scratch_bytes, void *stream);
 
+/*
+ * The same computation as cs_cosine_greedy without any workspace argument and
+ * without allocating: the workspace is a file-scope device array of
+ * CS_NOALLOC_WORKSPACE_BYTES that every GPU the library runs on holds until
+ * the process ends (its device address is looked up once per device, on the
+ * first call with that device current).  The caller passes inputs and
+ * outputs; the call makes no hipMalloc / hipFree or other allocating call.
+ * Query sets whose cs_cosine_greedy_workspace_size exceeds the array are
+ * rejected with CS_ERR_UNSUPPORTED.  Every call on one GPU shares that GPU's
+ * array and leaves its contents behind (a PREPARE-only call's index stays for
+ * later LIGHT | HEAVY calls until another call prepares): calls that overlap
+ * on one GPU, on any streams, race.
+ * No counterpart in the reference: this entry point is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid allocation).
+ */
+#define CS_NOALLOC_WORKSPACE_BYTES (64u << 20)
+
+cs_status cs_cosine_greedy_noalloc(const cs_spectra *refs, const cs_spectra *queries,
+                                   const cs_cosine_params *params, float *score, int32_t *matches,
+                                   uint8_t *overflow, int64_t ld, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batches of independent CosineGreedy matrices                               */
 /* ------------------------------------------------------------------------ */
