@@ -41,6 +41,7 @@ EXPORTED_SYMBOLS = (
     "cs_threshold_compact",
     "cs_validate_spectra",
     "cs_cosine_greedy_noalloc",
+    "cs_pack_records",
 )
 
 NOALLOC_WORKSPACE_BYTES = 64 << 20  # CS_NOALLOC_WORKSPACE_BYTES
@@ -178,6 +179,8 @@ def load(path: str = LIB_PATH):
     L.cs_cosine_greedy_noalloc.restype = i32
     L.cs_cosine_greedy_noalloc.argtypes = [ctypes.POINTER(CsSpectra), ctypes.POINTER(CsSpectra),
                                            ctypes.POINTER(CsCosineParams), p, p, p, i64, p]
+    L.cs_pack_records.restype = i32
+    L.cs_pack_records.argtypes = [p, p, p, i64, i64, i64, p, p]
     L.cs_validate_spectra.restype = i32
     L.cs_validate_spectra.argtypes = [ctypes.POINTER(CsSpectra), p, p]
     _lib = L

@@ -109,6 +109,24 @@ def cosine_greedy_noalloc(refs, queries, tolerance: float, shift: float, match_l
                                              _stream_ptr(stream)))
 
 
+def pack_records(score: torch.Tensor, matches: torch.Tensor, overflow: torch.Tensor,
+                 records: torch.Tensor, rows: Optional[int] = None, cols: Optional[int] = None,
+                 stream=None) -> None:
+    """Interleave the three planes into packed 9-byte records (score f32,
+    matches i32, overflow u8) in the uint8 device tensor ``records``."""
+    require_gpu()
+    L = _native.load()
+    rows = score.shape[0] if rows is None else rows
+    cols = score.shape[1] if cols is None else cols
+    if records.dtype != torch.uint8 or not records.is_contiguous() or records.numel() < 9 * rows * cols:
+        raise ValueError("records must be a contiguous uint8 tensor of 9 * rows * cols bytes")
+    if matches.stride(0) != score.stride(0) or overflow.stride(0) != score.stride(0):
+        raise ValueError("the three planes must share one row stride")
+    _native.check(L.cs_pack_records(score.data_ptr(), matches.data_ptr(), overflow.data_ptr(),
+                                    rows, cols, score.stride(0), records.data_ptr(),
+                                    _stream_ptr(stream)))
+
+
 def validate_spectra(spectra, stream=None) -> int:
     """CS_FAULT_* bits of a device-resident spectrum set (0 = sound), from the
     device validator cs_validate_spectra; synchronises ``stream``."""

@@ -11,7 +11,7 @@ from typing import List, Literal
 
 import numpy as np
 
-from ..device import Workspace, cosine_greedy, require_gpu
+from ..device import Workspace, cosine_greedy, pack_records, require_gpu
 from ..sparse import new_sparse
 from ..spectra import DeviceSpectra, PackedSpectra
 from ..utils import CudaTimer
@@ -128,14 +128,31 @@ class CudaCosineGreedy(BaseSimilarity):
                           overflow, self._workspace, self.light_candidates, self.round_capacity)
             timer.stop()
             self.kernel_time = timer.elapsed_seconds()
-            return np.rec.fromarrays([score.cpu().numpy(), matches.cpu().numpy(),
-                                      overflow.cpu().numpy()], dtype=self.score_datatype)
+            return self._download_records(score, matches, overflow)
         return self._sparse(refs, ques, timer)
+
+    def _download_records(self, score, matches, overflow) -> np.ndarray:
+        """The three device planes as one np.recarray: interleaved into packed
+        9-byte records on the device, then one copy into pinned host memory
+        that the returned array keeps (no host-side interleave)."""
+        import torch
+
+        R, Q = score.shape
+        dtype = np.dtype(self.score_datatype)
+        assert dtype.itemsize == 9
+        if R * Q == 0:
+            return np.zeros((R, Q), dtype=dtype).view(np.recarray)
+        records = torch.empty(9 * R * Q, dtype=torch.uint8, device=score.device)
+        pack_records(score, matches, overflow, records)
+        host = torch.empty(9 * R * Q, dtype=torch.uint8, pin_memory=True)
+        host.copy_(records, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        return host.numpy().view(dtype).reshape(R, Q).view(np.recarray)
 
     def _sparse(self, refs: DeviceSpectra, ques: DeviceSpectra, timer):
         import torch
 
-        from ..compact import compact_threshold
+        from ..device import threshold_compact
 
         R, Q = refs.n_spectra, ques.n_spectra
         rows = max(1, min(R, int(self.batch_size)))
@@ -152,7 +169,7 @@ class CudaCosineGreedy(BaseSimilarity):
                           overflow, self._workspace, self.light_candidates, self.round_capacity)
             timer.stop()
             self.kernel_time += timer.elapsed_seconds()
-            parts.append(compact_threshold(score[: r1 - r0], matches[: r1 - r0],
+            parts.append(threshold_compact(score[: r1 - r0], matches[: r1 - r0],
                                            overflow[: r1 - r0], self.sparse_threshold, r0))
         r = np.concatenate([p[0] for p in parts])
         q = np.concatenate([p[1] for p in parts])

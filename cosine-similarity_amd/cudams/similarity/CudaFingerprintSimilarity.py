@@ -3,7 +3,9 @@ cudams.similarity.CudaFingerprintSimilarity (reference
 cudams/similarity/CudaFingerprintSimilarity.py:13-186).
 
 Fingerprints are bit-packed on the host (one bit per position, nonzero = 1)
-and compared on int8 matrix cores with exact int32 counts.
+and compared on the matrix cores: the packed words feed fp4 (e2m1) MFMA
+operands directly, one bit plane per scale, and the float32 accumulators hold
+exact intersection counts.
 """
 from __future__ import annotations
 

@@ -331,6 +331,18 @@ cs_status cs_threshold_compact(const float *score, const int32_t *matches,
                                int64_t *out_col, float *out_score, int32_t *out_matches,
                                uint8_t *out_overflow, void *stream);
 
+/*
+ * The dense result as packed 9-byte records, row-major without padding:
+ * records[(r*cols + c)*9 + 0..3] = score (float32), + 4..7 = matches (int32),
+ * + 8 = overflow (uint8), little-endian -- the memory of a numpy array of
+ * dtype [("score", "<f4"), ("matches", "<i4"), ("overflow", "u1")], so one
+ * device-to-host copy delivers the structured array.  `records` must be
+ * 4-byte aligned.  Replaces the three copies and np.rec.fromarrays of
+ * CudaCosineGreedy.py:201-205.
+ */
+cs_status cs_pack_records(const float *score, const int32_t *matches, const uint8_t *overflow,
+                          int64_t rows, int64_t cols, int64_t ld, uint8_t *records, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Host engine (libcsamd_cpu.so: g++ + OpenMP, no GPU needed)                 */
 /* ------------------------------------------------------------------------ */

@@ -1,4 +1,4 @@
-"""Fingerprint similarity on the GPU (int8 MFMA) against the oracle.  Bar:
+"""Fingerprint similarity on the GPU (fp4 bit-plane MFMA) against the oracle.  Bar:
 intersection counts bit-exact; jaccard/dice bit-exact float32 of the float64
 quotient; cosine within 1e-6."""
 import json

@@ -127,3 +127,28 @@ def test_device_validator_flags_each_fault(gpu, bad):
             check_spectra_on_device(S)
     else:
         check_spectra_on_device(S)
+
+
+@pytest.mark.parametrize("rows,cols,pad", [(1, 1, 0), (7, 5, 3), (33, 127, 1), (256, 300, 0)])
+def test_pack_records_matches_numpy_structured_array(gpu, rows, cols, pad):
+    """cs_pack_records gives byte for byte the memory numpy builds for the
+    structured dtype from the three planes (padded leading dimension, record
+    counts that are not a multiple of four)."""
+    import torch
+
+    from cudams.device import pack_records
+
+    rng = np.random.default_rng(rows * 1000 + cols)
+    ld = cols + pad
+    s = rng.standard_normal((rows, ld)).astype(np.float32)
+    m = rng.integers(-2 ** 31, 2 ** 31 - 1, (rows, ld)).astype(np.int32)
+    o = rng.integers(0, 256, (rows, ld)).astype(np.uint8)
+    dev = [torch.from_numpy(x).cuda() for x in (s, m, o)]
+    out = torch.full((9 * rows * cols + 3,), 0xAB, dtype=torch.uint8, device="cuda")
+    print(f"pack_records {rows}x{cols} ld {ld}", flush=True)
+    pack_records(dev[0][:, :cols], dev[1][:, :cols], dev[2][:, :cols], out, rows, cols)
+    got = out.cpu().numpy()
+    want = np.rec.fromarrays([s[:, :cols], m[:, :cols], o[:, :cols]],
+                             dtype=[("score", "<f4"), ("matches", "<i4"), ("overflow", "u1")])
+    assert got[:9 * rows * cols].tobytes() == want.tobytes()
+    assert (got[9 * rows * cols:] == 0xAB).all()
