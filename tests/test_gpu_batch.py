@@ -170,3 +170,42 @@ def test_autotune_picks_a_timed_shape(gpu, tmp_path):
     assert best in timings and len(timings) == 2
     again, t2 = autotune(jobs, cache_path=cache)
     assert again == best and t2 == {}
+
+
+def test_batched_jobs_workload_vs_oracle(gpu, oracle):
+    """The 64-job workload at its stated size (seed 6: R x Q log-uniform in
+    [16, 2048] x [16, 4096], tolerance 0.01 or 0.1, mz_power 0.5; 14,966,214
+    pairs), captured in a HIP graph and replayed; every job of the replay
+    against the oracle: matches and overflow exact, score within 1e-5."""
+    import os as _os
+
+    import torch
+
+    from cudams import synthetic
+    from cudams.batch import BatchRunner
+
+    jobs = synthetic.batched_jobs(seed=6)
+    assert len(jobs) == 64
+    say("64 batched jobs: capture + replay")
+    runner = BatchRunner(jobs)
+    runner.run()
+    runner.capture()
+    for d in runner.djobs:
+        d.matches.fill_(-3)
+    torch.cuda.synchronize()
+    runner.replay()
+    got = runner.results()
+    runner.close()
+    cores = len(_os.sched_getaffinity(0))
+    pairs = 0
+    for k, (j, (s, m, o)) in enumerate(zip(jobs, got)):
+        w = oracle.cosine_greedy_matrix((j.refs.mz, j.refs.intensities, j.refs.offsets),
+                                        (j.queries.mz, j.queries.intensities, j.queries.offsets),
+                                        tolerance=j.tolerance, mz_power=j.mz_power,
+                                        intensity_power=j.intensity_power, nthreads=cores)
+        np.testing.assert_array_equal(m, w["matches"], err_msg=f"job {k}")
+        np.testing.assert_array_equal(o, (w["n_candidates"] >= j.match_limit).astype(np.uint8),
+                                      err_msg=f"job {k}")
+        np.testing.assert_allclose(s, w["score"], rtol=0, atol=SCORE_ATOL, err_msg=f"job {k}")
+        pairs += m.size
+    assert pairs == 14_966_214

@@ -126,3 +126,34 @@ def test_full_size_sampled(gpu, oracle):
     fingerprint_similarity("jaccard", wb, wa, out, intersections=inter2)
     torch.cuda.synchronize()
     assert torch.equal(inter, inter2.t())
+
+
+@pytest.mark.parametrize("measure", ["jaccard", "dice", "cosine"])
+def test_full_size_every_pair(gpu, oracle, measure):
+    """The whole 4096 x 4096 x 2048-bit workload (seeds 3 and 4) for each
+    measure against the oracle over every pair: intersections exact,
+    jaccard and dice the float32 of the float64 quotient, cosine within
+    1e-6."""
+    import os as _os
+
+    import torch
+
+    from cudams import synthetic
+    from cudams.device import fingerprint_similarity
+
+    a = synthetic.fingerprints(4096, 3)
+    b = synthetic.fingerprints(4096, 4)
+    wa = torch.from_numpy(synthetic.pack_bits(a).view(np.int32)).cuda()
+    wb = torch.from_numpy(synthetic.pack_bits(b).view(np.int32)).cuda()
+    out = torch.empty((4096, 4096), dtype=torch.float32, device="cuda")
+    inter = torch.empty((4096, 4096), dtype=torch.int32, device="cuda")
+    print(f"fingerprints 4096x4096x2048 {measure}", flush=True)
+    fingerprint_similarity(measure, wa, wb, out, intersections=inter)
+    torch.cuda.synchronize()
+    want, wi = oracle.fingerprint_matrix(a, b, measure, nthreads=len(_os.sched_getaffinity(0)))
+    np.testing.assert_array_equal(inter.cpu().numpy(), wi)
+    got = out.cpu().numpy()
+    if measure == "cosine":
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-6)
+    else:
+        np.testing.assert_array_equal(got, want.astype(np.float32))

@@ -119,3 +119,54 @@ def test_server_rejects_oversized_request(gpu):
     big = synthetic.lognormal_spectra(5, 4, cluster_fraction=0.0)
     with pytest.raises(ValueError):
         srv.submit(big)
+
+
+def test_served_library_workload_vs_oracle(gpu, oracle):
+    """The served workload at its stated size: a 500,000-spectrum library
+    (lognormal peaks, seed 7) indexed once; requests of 1, 16 and 64 spectra
+    (query pool seed 8) served together through the continuous-batching
+    server and one by one through the plain path.  Both give identical hits;
+    against the oracle over the whole library, each row's ten scores are the
+    ten best within 1e-5 (no library spectrum outside the list scores higher),
+    their matches exact, and in order (score desc, library index asc)."""
+    import os as _os
+
+    from cudams import synthetic
+    from cudams.serving import LibraryIndex, PlainSearch, SearchServer, pack_request
+    from cudams.spectra import PackedSpectra
+
+    lib = synthetic.lognormal_spectra(500_000, 7, cluster_fraction=0.0)
+    pool = synthetic.lognormal_spectra(4096, 8, cluster_fraction=0.0)
+    say("500k library: index")
+    ix = LibraryIndex(lib, tolerance=0.01)
+    reqs = []
+    for start, size in ((0, 1), (100, 16), (1000, 64)):
+        off = pool.offsets[start:start + size + 1]
+        reqs.append(PackedSpectra(pool.mz[off[0]:off[-1]], pool.intensities[off[0]:off[-1]],
+                                  off - off[0]))
+    say("served")
+    with SearchServer(ix, k=10, slots=3) as srv:
+        futs = [srv.submit(pack_request(p)) for p in reqs]
+        served = [f.result(timeout=300) for f in futs]
+    say("plain")
+    plain = PlainSearch(ix, k=10)
+    alone = [plain.search(pack_request(p)) for p in reqs]
+    csr = (lib.mz, lib.intensities, lib.offsets)
+    cores = len(_os.sched_getaffinity(0))
+    for p, a, b in zip(reqs, alone, served):
+        np.testing.assert_array_equal(a.indices, b.indices)
+        np.testing.assert_array_equal(a.scores, b.scores)
+        np.testing.assert_array_equal(a.matches, b.matches)
+        say(f"oracle: {p.n_spectra} x 500000")
+        want = oracle.cosine_greedy_matrix((p.mz, p.intensities, p.offsets), csr,
+                                           tolerance=0.01, nthreads=cores)
+        for row in range(p.n_spectra):
+            s, idx = want["score"][row], a.indices[row]
+            assert (idx >= 0).all()
+            np.testing.assert_allclose(a.scores[row], s[idx], rtol=0, atol=SCORE_ATOL)
+            np.testing.assert_array_equal(a.matches[row], want["matches"][row][idx])
+            best = np.sort(s)[::-1][:10]
+            np.testing.assert_allclose(a.scores[row], best, rtol=0, atol=SCORE_ATOL)
+            assert np.all(np.delete(s, idx) <= a.scores[row][-1] + SCORE_ATOL)
+            order = np.lexsort((idx, -a.scores[row]))
+            assert np.array_equal(order, np.arange(10))
