@@ -28,7 +28,7 @@
 // consecutive queries per lane (one 16-byte non-temporal store) with the
 // measure computed in float32 (jaccard, dice: the correctly rounded quotient
 // of exact integers, which equals the float32 rounding of the float64
-// quotient) or float64 (cosine).
+// quotient; cosine: the count times a reciprocal square root).
 //
 // On gfx950 VALU instructions and MFMAs of one SIMD do not overlap, so the
 // kernel is priced as MFMA cycles + VALU cycles: every VALU op in the K loop
@@ -127,14 +127,37 @@ __device__ __forceinline__ float div_rn(float n, float d) {
     return fmaf(t, r, q);
 }
 
+// n / d for integers 0 <= n <= d <= kShortDivMax: the hardware reciprocal,
+// the quotient, and one residual correction rounding once.  On this domain
+// the result is the correctly rounded quotient: the GPU tests check every
+// (n, d) pair of it against the float32 rounding of the float64 quotient
+// (fingerprints with prefix bit patterns give intersection min(a, b) and
+// denominators max(a, b) and a + b for all a, b <= 4096).
+constexpr int64_t kShortDivBits = 4096;  // fingerprints up to this length use div_short
+__device__ __forceinline__ float div_short(float n, float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float q = n * r;
+    const float t = fmaf(-d, q, n);
+    return fmaf(t, r, q);
+}
+
 // Measure from the exact counts (float32 integers up to 2^23 bits, so every
 // sum below is exact).  A zero denominator only occurs with inter = 0, where
-// dividing by 1 gives the reference's nan_to_num 0.
-template <int kMeasure>
+// dividing by 1 gives the reference's nan_to_num 0.  Cosine, whose bar is a
+// tolerance (the reference computes it in float32, vector_similarity_
+// functions.py:116), is the count times the hardware reciprocal square root
+// of a * b: within a few float32 ulps of the float64 value.
+template <int kMeasure, bool kShort>
 __device__ __forceinline__ float measure_value(float inter, float a, float b) {
-    if (kMeasure == CS_JACCARD) return div_rn(inter, fmaxf((a - inter) + b, 1.0f));
-    if (kMeasure == CS_DICE) return 2.0f * div_rn(inter, fmaxf(a + b, 1.0f));
-    return (float)((double)inter / sqrt(fmax((double)a * (double)b, 1.0)));
+    if (kMeasure == CS_JACCARD) {
+        const float u = fmaxf((a - inter) + b, 1.0f);
+        return kShort ? div_short(inter, u) : div_rn(inter, u);
+    }
+    if (kMeasure == CS_DICE) {
+        const float u = fmaxf(a + b, 1.0f);
+        return 2.0f * (kShort ? div_short(inter, u) : div_rn(inter, u));
+    }
+    return inter * __builtin_amdgcn_rsqf(fmaxf(a * b, 1.0f));
 }
 
 template <int P>
@@ -196,7 +219,7 @@ __device__ __forceinline__ void set_tile(int64_t tile, int64_t gx, int wave, int
 }
 
 // Measures of one wave tile from its accumulators and lane popcounts.
-template <int kMeasure>
+template <int kMeasure, bool kShort>
 __device__ __forceinline__ void fp_epilogue(const v4f (&acc)[kQF][kRF], uint32_t (&pr)[kRF],
                                             uint32_t (&pq)[kQF], int64_t rb, int64_t qb,
                                             int lr, int grp, int64_t nR, int64_t nQ,
@@ -238,7 +261,7 @@ __device__ __forceinline__ void fp_epilogue(const v4f (&acc)[kQF][kRF], uint32_t
             v4f v;
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                v[e] = measure_value<kMeasure>(acc[qf][rf][e], a, qcount[qf][e]);
+                v[e] = measure_value<kMeasure, kShort>(acc[qf][rf][e], a, qcount[qf][e]);
             if (flagged) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -270,7 +293,7 @@ __device__ __forceinline__ void fp_epilogue(const v4f (&acc)[kQF][kRF], uint32_t
 // tiles per row of tiles).  The first step of the next tile is loaded during
 // the last step of the current one, so its latency hides behind the MFMAs
 // and the epilogue.
-template <bool kFull, int kMeasure>
+template <bool kFull, int kMeasure, bool kShort>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_fp_gemm(const uint32_t *__restrict__ R, int64_t nR, const uint32_t *__restrict__ Q, int64_t nQ,
           int64_t n_words, int64_t gx, int64_t n_tiles, const uint8_t *__restrict__ ref_valid,
@@ -315,7 +338,7 @@ k_fp_gemm(const uint32_t *__restrict__ R, int64_t nR, const uint32_t *__restrict
             }
             fp_step(r1, q1, acc, pr, pq);
         }
-        fp_epilogue<kMeasure>(acc, pr, pq, rb, qb, lr, grp, nR, nQ, ref_valid, query_valid,
+        fp_epilogue<kMeasure, kShort>(acc, pr, pq, rb, qb, lr, grp, nR, nQ, ref_valid, query_valid,
                               empty_score, out, inter_out, ld, vec_store);
         if (next >= n_tiles) break;
         tile = next;
@@ -387,14 +410,22 @@ extern "C" cs_status cs_fingerprint_similarity(int measure, const uint32_t *ref_
                            n_refs, query_words, n_queries, n_words, gx, n_tiles, ref_valid,
                            query_valid, empty_score, out, intersections, ld, vec);
     };
+    // the short division is exact for denominators up to 2 * 4096 (dice)
+    const bool shrt = n_words * 32 <= kShortDivBits;
     if (full) {
-        if (measure == CS_JACCARD) launch(k_fp_gemm<true, CS_JACCARD>);
-        else if (measure == CS_DICE) launch(k_fp_gemm<true, CS_DICE>);
-        else launch(k_fp_gemm<true, CS_COSINE>);
+        if (measure == CS_JACCARD)
+            shrt ? launch(k_fp_gemm<true, CS_JACCARD, true>) : launch(k_fp_gemm<true, CS_JACCARD, false>);
+        else if (measure == CS_DICE)
+            shrt ? launch(k_fp_gemm<true, CS_DICE, true>) : launch(k_fp_gemm<true, CS_DICE, false>);
+        else
+            launch(k_fp_gemm<true, CS_COSINE, false>);
     } else {
-        if (measure == CS_JACCARD) launch(k_fp_gemm<false, CS_JACCARD>);
-        else if (measure == CS_DICE) launch(k_fp_gemm<false, CS_DICE>);
-        else launch(k_fp_gemm<false, CS_COSINE>);
+        if (measure == CS_JACCARD)
+            shrt ? launch(k_fp_gemm<false, CS_JACCARD, true>) : launch(k_fp_gemm<false, CS_JACCARD, false>);
+        else if (measure == CS_DICE)
+            shrt ? launch(k_fp_gemm<false, CS_DICE, true>) : launch(k_fp_gemm<false, CS_DICE, false>);
+        else
+            launch(k_fp_gemm<false, CS_COSINE, false>);
     }
     CS_LAUNCH_CHECK("k_fp_gemm");
     return CS_OK;

@@ -157,3 +157,44 @@ def test_full_size_every_pair(gpu, oracle, measure):
         np.testing.assert_allclose(got, want, rtol=0, atol=1e-6)
     else:
         np.testing.assert_array_equal(got, want.astype(np.float32))
+
+
+@pytest.mark.parametrize("measure", ["jaccard", "dice", "cosine"])
+def test_every_count_pair_up_to_4096_bits(gpu, measure):
+    """Fingerprints of 4096 bits whose set bits are prefixes: reference a and
+    query b (a, b = 0..4096) intersect in min(a, b) bits, so the 4097 x 4097
+    matrix holds every (intersection, denominator) pair the measures can meet
+    at this length: jaccard min/max, dice 2 min/(a + b), every n <= d.  The
+    division the kernel uses up to 4096 bits must give the float32 rounding of
+    the float64 quotient on all of them (bit for bit); cosine is within
+    1e-6."""
+    import torch
+
+    from cudams import synthetic
+    from cudams.device import fingerprint_similarity
+
+    n, bits = 4097, 4096
+    fp = (np.arange(bits)[None, :] < np.arange(n)[:, None]).astype(np.uint8)
+    w = torch.from_numpy(synthetic.pack_bits(fp).view(np.int32)).cuda()
+    out = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    inter = torch.empty((n, n), dtype=torch.int32, device="cuda")
+    print(f"prefix fingerprints {n}x{n}x{bits} {measure}", flush=True)
+    fingerprint_similarity(measure, w, w, out, intersections=inter)
+    torch.cuda.synchronize()
+    a = np.arange(n, dtype=np.float64)
+    lo = np.minimum.outer(a, a)
+    np.testing.assert_array_equal(inter.cpu().numpy(), lo.astype(np.int32))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        if measure == "jaccard":
+            want = lo / np.maximum.outer(a, a)
+        elif measure == "dice":
+            want = 2.0 * (lo / np.add.outer(a, a))
+        else:
+            want = lo / np.sqrt(np.multiply.outer(a, a))
+    want = np.nan_to_num(want)
+    got = out.cpu().numpy()
+    if measure == "cosine":
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-6)
+    else:
+        bad = got != want.astype(np.float32)
+        assert int(bad.sum()) == 0, f"{int(bad.sum())} pairs differ, e.g. {np.argwhere(bad)[:5]}"
