@@ -11,8 +11,15 @@ call (query tiling + lane kernel + cooperative kernel), issued as its three
 stages with HIP events between them, producing score/matches/overflow for all
 67,108,864 pairs.
 
-Prints one JSON line (rank 0).  ``--workload fingerprint`` runs the 4096 x
-4096 x 2048-bit Jaccard instead.
+Prints one JSON line (rank 0).  Beside the step it carries: ``roofline`` of
+the lane kernel (compulsory bytes against HBM, VALU busy from the committed
+counter profile), ``cpu_baseline`` (the oracle over the whole matrix on the
+host cores), ``parity`` (every pair against that oracle, near ties counted),
+``matrix_wall`` (the drop-in ``CudaCosineGreedy.matrix`` call end to end),
+``fingerprint`` (the three measures at 4096 x 4096 x 2048 bits),
+``batched_jobs`` (64 uneven jobs, graph vs one call per job) and ``serving``
+(500,000-spectrum library, Poisson request streams, served vs plain).
+``--workload fingerprint`` runs the fingerprint section as its own line.
 """
 from __future__ import annotations
 
@@ -417,7 +424,9 @@ def bench_serving(a) -> dict:
     run_load("served", ix, reqs, arr)
     run_load("plain", ix, reqs, arr)
     rates = []
-    for rate in (50.0, 200.0, 1000.0):
+    # the workload's three rates, then a sustained rate between plain's and
+    # served's saturation throughput (about 1,300 and 2,700 requests/s)
+    for rate in (50.0, 200.0, 1000.0, 2000.0):
         reqs, arr = stream(rate, a.serve_seconds, int(rate))
         row = {"rate_per_s": rate, "seconds": a.serve_seconds}
         for mode in ("served", "plain"):
