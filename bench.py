@@ -16,6 +16,7 @@ the lane kernel (compulsory bytes against HBM, VALU busy from the committed
 counter profile), ``cpu_baseline`` (the oracle over the whole matrix on the
 host cores), ``parity`` (every pair against that oracle, near ties counted),
 ``matrix_wall`` (the drop-in ``CudaCosineGreedy.matrix`` call end to end),
+``cpu_parallel`` (the host engine behind ``CPUParallelCosineGreedy``),
 ``fingerprint`` (the three measures at 4096 x 4096 x 2048 bits),
 ``batched_jobs`` (64 uneven jobs, graph vs one call per job) and ``serving``
 (500,000-spectrum library, Poisson request streams, served vs plain).
@@ -247,6 +248,8 @@ def bench_cosine(a) -> dict:
         log("parity against the oracle")
         res["parity"] = cosine_parity(got, want, packed, tol, ml)
         log(f"parity {res['parity']}")
+    if not a.no_cpu_parallel:
+        res["cpu_parallel"] = bench_cpu_parallel(packed, tol, want)
     del got, want
     if not a.no_matrix_wall:
         res["matrix_wall"] = bench_matrix_wall(a, packed)
@@ -328,6 +331,36 @@ def bench_fingerprint(a) -> dict:
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp4",
             "data": "synthetic", "config": fp["config"], "roofline": j["roofline"],
             "cpu_baseline": j.get("cpu_baseline"), "fingerprint": fp}
+
+
+def bench_cpu_parallel(packed, tol, want) -> dict:
+    """CPUParallelCosineGreedy (the host engine, libcsamd_cpu.so) on the whole
+    matrix with this host's cores, wall clock from spectrum objects to the
+    structured array, and its agreement with the oracle when that matrix is
+    at hand.  The reference keeps its CPU class for comparison with the GPU
+    one (spectrum_similarity_functions.py:290-293)."""
+    import numpy as np
+
+    from cudams.similarity import CPUParallelCosineGreedy
+    from cudams.spectrum import Spectrum
+
+    off = packed.offsets
+    spectra = [Spectrum(packed.mz[off[i]:off[i + 1]], packed.intensities[off[i]:off[i + 1]])
+               for i in range(packed.n_spectra)]
+    cores = host_threads()
+    cpu = CPUParallelCosineGreedy(tolerance=tol, n_threads=cores)
+    t0 = time.perf_counter()
+    out = cpu.matrix(spectra, spectra)
+    t = time.perf_counter() - t0
+    n = packed.n_spectra
+    rec = {"what": f"CPUParallelCosineGreedy(tolerance={tol}).matrix(spectra, spectra)",
+           "value": n * n / t, "unit": "pairs/s", "cores": cores, "seconds": t}
+    if want is not None:
+        rec["matches_mismatch"] = int((out["matches"] != want["matches"]).sum())
+        rec["max_abs_score_err"] = float(
+            np.abs(out["score"].astype(np.float64) - want["score"]).max())
+    log(f"cpu engine {t:.2f} s on {cores} threads: {rec}")
+    return rec
 
 
 def bench_matrix_wall(a, packed) -> dict:
@@ -540,6 +573,8 @@ def main() -> int:
                     help="skip the end-to-end timing of the drop-in matrix() call")
     ap.add_argument("--no-fingerprint", action="store_true",
                     help="skip the fingerprint section of the cosine line")
+    ap.add_argument("--no-cpu-parallel", action="store_true",
+                    help="skip the CPUParallelCosineGreedy timing of the cosine line")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-jobs section of the cosine line")
     ap.add_argument("--no-serving", action="store_true",

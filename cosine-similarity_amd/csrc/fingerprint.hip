@@ -127,12 +127,13 @@ __device__ __forceinline__ float div_rn(float n, float d) {
     return fmaf(t, r, q);
 }
 
-// n / d for integers 0 <= n <= d <= kShortDivMax: the hardware reciprocal,
-// the quotient, and one residual correction rounding once.  On this domain
-// the result is the correctly rounded quotient: the GPU tests check every
-// (n, d) pair of it against the float32 rounding of the float64 quotient
-// (fingerprints with prefix bit patterns give intersection min(a, b) and
-// denominators max(a, b) and a + b for all a, b <= 4096).
+// n / d for the (intersection, denominator) pairs of fingerprints of at most
+// kShortDivBits bits: the hardware reciprocal, the quotient, and one residual
+// correction rounding once.  On this domain the result is the correctly
+// rounded quotient: the GPU tests check every such pair against the float32
+// rounding of the float64 quotient (fingerprints with prefix bit patterns give
+// intersection min(a, b) with the jaccard denominator max(a, b) and the dice
+// denominator a + b, for all a, b <= 4096).
 constexpr int64_t kShortDivBits = 4096;  // fingerprints up to this length use div_short
 __device__ __forceinline__ float div_short(float n, float d) {
     const float r = __builtin_amdgcn_rcpf(d);

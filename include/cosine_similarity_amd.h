@@ -361,8 +361,8 @@ const char *cs_cpu_last_error(void);
  * returns CS_ERR_INVALID.  n_threads <= 0 uses OpenMP's default.
  * Replaces the numba prange kernel cpu_parallel_cosine_greedy_kernel
  * (cudams/similarity/spectrum_similarity_functions.py:281-320) called from
- * CPUParallelCosineGreedy._matrix_with_numpy_output
- * (cudams/similarity/CPUParallelCosineGreedy.py:50-70).
+ * CPUParallelCosineGreedy (the kernel built at
+ * cudams/similarity/CPUParallelCosineGreedy.py:42-49, called at :51-68).
  */
 cs_status cs_cpu_cosine_greedy(const cs_spectra *refs, const cs_spectra *queries,
                                double tolerance, double shift, float *score, int32_t *matches,
