@@ -48,6 +48,13 @@ CS_HD bool exact_window(double mz_r, double mz_q, double tol, double shift) {
     return !(x > mz_r + tol) & !(x < mz_r - tol);
 }
 
+// The same test for shift == 0: mz_q + 0.0 equals mz_q for every mz_q but
+// -0.0 (which becomes +0.0), and the comparisons do not tell the two apart,
+// so the addition can be left out.
+CS_HD bool exact_window_noshift(double mz_r, double mz_q, double tol) {
+    return !(mz_q > mz_r + tol) & !(mz_q < mz_r - tol);
+}
+
 // Up to K candidates kept sorted by key_greater (descending) in registers.
 // All indexing is compile-time after unrolling, so the arrays stay in VGPRs.
 // The unrolled steps past the wave's bound are left by an early exit for
@@ -85,6 +92,25 @@ struct LaneCandidates {
             cij = tij;
         }
         ++n;
+    }
+
+    // Insert the candidate of rank s when the s before it are already held
+    // (n == s): it is compared with positions 0 .. s - 1 only and the last
+    // one moves to position s.  With s a constant after unrolling, inserting
+    // B candidates costs B (B - 1) / 2 compare-exchanges instead of B * B.
+    CS_HD void insert_rank(int s, double cv, uint32_t cij) {
+#pragma unroll
+        for (int p = 0; p < K; ++p) {
+            if (p > s) break;
+            const bool take = (p == s) | key_greater(cv, cij, v[p], ij[p]);
+            const double tv = take ? v[p] : cv;
+            const uint32_t tij = take ? ij[p] : cij;
+            v[p] = take ? cv : v[p];
+            ij[p] = take ? cij : ij[p];
+            cv = tv;
+            cij = tij;
+        }
+        n = s + 1;
     }
 
     // Greedy unique assignment over the sorted candidates.  `bound` >= n may

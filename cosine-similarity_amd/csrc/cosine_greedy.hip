@@ -809,6 +809,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
 
     const int64_t q = q0 + lane;
     const bool valid = q < a.nQ;
+    const unsigned ulane = (unsigned)lane;
 
     auto grab = [&]() {
         int v = 0;
@@ -826,11 +827,15 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             c.n = uniform(roff_l[rr + 1]) - b0;
             c.norm = R.norm[r0 + rr];
         }
+        // uniform base + 32-bit lane offset: the scalar-base address form,
+        // no per-lane 64-bit address arithmetic
+        const float *rmz = R.mz + c.off;
+        const double *rin = R.inp + c.off;
         const bool h0 = lane < c.n, h1 = lane + kWave < c.n;
-        c.mz0 = h0 ? R.mz[c.off + lane] : 0.0f;
-        c.mz1 = h1 ? R.mz[c.off + kWave + lane] : 0.0f;
-        c.in0 = h0 ? R.inp[c.off + lane] : 0.0;
-        c.in1 = h1 ? R.inp[c.off + kWave + lane] : 0.0;
+        c.mz0 = h0 ? rmz[ulane] : 0.0f;
+        c.mz1 = h1 ? rmz[kWave + ulane] : 0.0f;
+        c.in0 = h0 ? rin[ulane] : 0.0;
+        c.in1 = h1 ? rin[kWave + ulane] : 0.0;
     };
 
     // outputs of the previous reference, held in registers and stored only
@@ -842,10 +847,10 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     int out_overflow = 0;
     auto flush = [&]() {
         if (out_row >= 0 && valid) {
-            const int64_t o = out_row * a.ld + q;
-            a.score[o] = out_score;
-            a.matches[o] = out_matches;
-            a.overflow[o] = (uint8_t)out_overflow;
+            const int64_t o = out_row * a.ld + q0;  // uniform: the row's tile segment
+            (a.score + o)[ulane] = out_score;
+            (a.matches + o)[ulane] = out_matches;
+            (a.overflow + o)[ulane] = (uint8_t)out_overflow;
         }
     };
 
@@ -871,15 +876,14 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                 const float x0 = ri - a.tolc, x1 = ri + a.tolc;
                 auto emit = [&](int ql, int j, float qv, float qin) {
                     if (!((filter >> ql) & 1ull)) return;
-                    double mq, mr;
-                    if constexpr (F32) {
-                        mq = (double)qv;
-                        mr = (double)ri;
+                    if constexpr (F32) {  // no shift on this path
+                        if (!exact_window_noshift((double)ri, (double)qv, a.tol)) return;
                     } else {
-                        mq = a.exact_from_tile ? (double)qv : exact_mz(Q, qoff_l[ql] + j);
-                        mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+                        const double mq = a.exact_from_tile ? (double)qv
+                                                            : exact_mz(Q, qoff_l[ql] + j);
+                        const double mr = R.mzx ? R.mzx[roff + i] : (double)ri;
+                        if (!exact_window(mr, mq, a.tol, a.shift)) return;
                     }
-                    if (!exact_window(mr, mq, a.tol, a.shift)) return;
                     // rank within the query and pool slot, both atomics in flight at once
                     const int rank = atomicAdd(&cnt[ql], 1);
                     const int pos = atomicAdd(pool_n, 1);
@@ -1040,16 +1044,16 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             LaneCandidates<K> cand;
             cand.reset();
 #pragma unroll
-            for (int s = 0; s < K; ++s) {  // independent gathers through the rank index
+            for (int s = 0; s < K; ++s) {  // gathers through the rank index
                 if (s >= bound) {  // uniform
                     if constexpr (kLaneEarlyExit<K>)
                         break;
                     else
                         continue;
                 }
-                if (small && s < n) {
+                if (small && s < n) {  // rank s arrives after ranks 0 .. s - 1
                     const int e = pidx[s * kTileQueries + lane];
-                    cand.insert(pv[e], pij[e], bound);
+                    cand.insert_rank(s, pv[e], pij[e]);
                 }
             }
             if (small) {
