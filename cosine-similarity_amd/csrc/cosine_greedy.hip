@@ -551,30 +551,8 @@ constexpr size_t kLdsBytes = 160 * 1024;
 constexpr int kLightCap = (int)((kLdsBytes - light_lds_bytes(0) - 64) / kLightPeakBytes) - 1;
 static_assert(light_lds_bytes(kLightCap) <= kLdsBytes, "lane kernel LDS fits a workgroup");
 
-// Descending bitonic sort of one key per lane across the wave (64 lanes).
-// x from lane (lane ^ J).  J <= 8 stays within a 16-lane row and uses DPP
-// lane moves (VALU, no LDS round trip): quad_perm [1,0,3,2] and [2,3,0,1] for
-// 1 and 2, row_half_mirror then quad_perm [3,2,1,0] for 4 (lane i of a
-// half-row reads 7 - i, then quad lane k reads 3 - k: i ^ 4), row_ror:8 for
-// 8.  J = 16 is a ds_swizzle xor within 32 lanes, J = 32 a bpermute.
-template <int J>
-__device__ __forceinline__ int xor_lane(int x) {
-    if constexpr (J == 1) return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);
-    if constexpr (J == 2) return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);
-    if constexpr (J == 4)
-        return __builtin_amdgcn_update_dpp(
-            0, __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false), 0x1B, 0xF, 0xF, false);
-    if constexpr (J == 8) return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);
-    if constexpr (J == 16) return __builtin_amdgcn_ds_swizzle(x, 0x401F);
-    return __shfl_xor(x, 32);
-}
-
-template <int J>
-__device__ __forceinline__ double xor_lane_double(double x) {
-    const int lo = xor_lane<J>(__double2loint(x));
-    const int hi = xor_lane<J>(__double2hiint(x));
-    return __hiloint2double(hi, lo);
-}
+// Descending bitonic sort of one key per lane across the wave (64 lanes),
+// partner values from xor_lane (cs_internal.h).
 
 // One compare-exchange stage of the bitonic network (block size Kb, partner
 // distance J).

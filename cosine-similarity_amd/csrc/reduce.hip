@@ -11,8 +11,11 @@ namespace cs {
 namespace {
 
 constexpr int kTopkThreads = 256;
-constexpr int kTopkChunk = 4096;   // columns one block ranks in the first pass
+constexpr int kTopkChunk = 4096;   // columns of one batch of loads (16 per thread)
 constexpr int kTopkLoads = kTopkChunk / kTopkThreads;
+constexpr int kTopkBlocks = 512;   // first-pass blocks aimed at: each ranks whole batches
+constexpr int kTopkMaxBatches = 16;
+constexpr int kTopkStage = kTopkLoads * 64;  // candidates a wave stages: one batch at most
 constexpr int kMaxK = 16;
 
 // Ranking of one hit: higher score first, then lower column.  NaN ranks last;
@@ -61,17 +64,47 @@ struct TopList {
     }
 };
 
-// Best head over the wave (every lane gets it).
+template <int J>
+__device__ __forceinline__ void best_step(float &bs, int &bc) {
+    const float os = xor_lane_float<J>(bs);
+    const int oc = xor_lane<J>(bc);
+    const bool t = hit_before(os, oc, bs, bc);
+    bs = t ? os : bs;
+    bc = t ? oc : bc;
+}
+
+// Best head over the wave (every lane gets it): a butterfly whose partner
+// moves are DPP and swizzle steps, one bpermute (the whole wave must be active).
 __device__ __forceinline__ void wave_best(float &bs, int &bc) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const float os = __shfl_xor(bs, off);
-        const int oc = __shfl_xor(bc, off);
-        if (hit_before(os, oc, bs, bc)) {
-            bs = os;
-            bc = oc;
-        }
-    }
+    best_step<1>(bs, bc);
+    best_step<2>(bs, bc);
+    best_step<4>(bs, bc);
+    best_step<8>(bs, bc);
+    best_step<16>(bs, bc);
+    best_step<32>(bs, bc);
+}
+
+// One stage of a descending bitonic sort of one float per lane (no NaN).
+template <int SIZE, int J>
+__device__ __forceinline__ void fsort_stage(float &x, int lane) {
+    const float o = xor_lane_float<J>(x);
+    const bool keep_max = ((lane & J) == 0) == ((lane & SIZE) == 0);
+    x = keep_max ? fmaxf(x, o) : fminf(x, o);
+    if constexpr (J > 1) fsort_stage<SIZE, J / 2>(x, lane);
+}
+template <int SIZE>
+__device__ __forceinline__ void fsort_sizes(float &x, int lane) {
+    fsort_stage<SIZE, SIZE / 2>(x, lane);
+    if constexpr (SIZE < 64) fsort_sizes<SIZE * 2>(x, lane);
+}
+
+// The k-th largest of the wave's 64 values (k uniform, 1 <= k <= 64; no
+// NaN): a descending bitonic sort across the lanes, then lane k - 1.  The
+// whole wave must be active.
+__device__ __forceinline__ float wave_kth_largest(float x, int k) {
+    const int lane = threadIdx.x & 63;
+    fsort_sizes<2>(x, lane);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), k - 1));
 }
 
 // Merges the threads' lists of one block; the block's k best (k <= K) end
@@ -127,35 +160,81 @@ __device__ __forceinline__ void write_hits(const float *fs, const int *fc, int k
     }
 }
 
-// Pass 1: block b ranks columns [chunk * kTopkChunk, ...) of one row, with
-// all of a thread's loads issued before its first comparison.  One chunk per
-// row: the result is final; otherwise the chunk's k best go to the parts.
+// Pass 1: block b ranks columns [chunk * ccols, (chunk + 1) * ccols) of one
+// row, in batches of kTopkChunk columns with all of a thread's loads of a
+// batch issued before its first comparison.  A chunk spans as many batches as
+// keep about kTopkBlocks blocks in the grid, so the block merge (the
+// latency-bound part) is paid once per chunk, not once per 4096 columns.  One
+// chunk per row: the result is final; otherwise the chunk's k best go to the
+// parts.
 template <int K>
 __global__ void __launch_bounds__(kTopkThreads)
 k_topk_chunks(const float *__restrict__ score, const int32_t *__restrict__ matches, int64_t rows,
-              int64_t cols, int64_t ld, int k, int64_t nchunk, float *__restrict__ part_s,
-              int *__restrict__ part_c, float *__restrict__ top_score,
+              int64_t cols, int64_t ld, int k, int64_t nchunk, int64_t ccols,
+              float *__restrict__ part_s, int *__restrict__ part_c, float *__restrict__ top_score,
               int32_t *__restrict__ top_index, int32_t *__restrict__ top_matches) {
     __shared__ float ws[kTopkThreads / 64 * K], fs[K];
     __shared__ int wc[kTopkThreads / 64 * K], fc[K];
-    const int tid = threadIdx.x;
+    __shared__ float st_v[kTopkThreads / 64][kTopkStage];
+    __shared__ int st_c[kTopkThreads / 64][kTopkStage];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int64_t b = blockIdx.x; b < rows * nchunk; b += gridDim.x) {
         const int64_t row = b / nchunk, chunk = b - row * nchunk;
-        const int64_t c0 = chunk * kTopkChunk;
+        const int64_t c0 = chunk * ccols;
+        const int64_t c1 = c0 + ccols < cols ? c0 + ccols : cols;
         const float *s = score + row * ld;
-        float v[kTopkLoads];
+        // pass 1: the lane's largest value (NaN skipped), a batch of loads in
+        // flight at a time
+        float mx = -__builtin_huge_valf();
+        for (int64_t base = c0; base < c1; base += kTopkChunk) {
+            float v[kTopkLoads];
 #pragma unroll
-        for (int u = 0; u < kTopkLoads; ++u) {
-            const int64_t c = c0 + u * kTopkThreads + tid;
-            v[u] = c < cols ? s[c] : 0.0f;
+            for (int u = 0; u < kTopkLoads; ++u) {
+                const int64_t c = base + u * kTopkThreads + tid;
+                v[u] = c < c1 ? s[c] : -__builtin_huge_valf();
+            }
+#pragma unroll
+            for (int u = 0; u < kTopkLoads; ++u) mx = fmaxf(mx, v[u]);
         }
+        // the k-th largest lane maximum bounds the block's k-th best from
+        // below (it is the k-th of k real values of the block), so only
+        // values at or above it can rank; below k real values (-inf) every
+        // value is a candidate, NaN included
+        const float tau = wave_kth_largest(mx, k);
+        const bool all = tau == -__builtin_huge_valf();
+        // pass 2: the candidates (the chunk is read again, from the cache)
+        // are staged in the wave's LDS list and inserted into the threads'
+        // lists from there, one insertion in the code and lanes busy with
+        // candidates instead of each lane with its own (few values pass)
         TopList<K> L;
         L.clear();
+        int staged = 0;  // uniform
+        auto drain = [&]() {
+            for (int e = lane; e < staged; e += 64) L.insert(st_v[wave][e], st_c[wave][e]);
+            staged = 0;
+        };
+        for (int64_t base = c0; base < c1; base += kTopkChunk) {
+            float v[kTopkLoads];
 #pragma unroll
-        for (int u = 0; u < kTopkLoads; ++u) {
-            const int64_t c = c0 + u * kTopkThreads + tid;
-            if (c < cols) L.insert(v[u], (int)c);
+            for (int u = 0; u < kTopkLoads; ++u) {
+                const int64_t c = base + u * kTopkThreads + tid;
+                v[u] = c < c1 ? s[c] : -__builtin_huge_valf();
+            }
+#pragma unroll
+            for (int u = 0; u < kTopkLoads; ++u) {
+                const int64_t c = base + u * kTopkThreads + tid;
+                const bool pass = c < c1 && (v[u] >= tau || all);
+                const unsigned long long m = __ballot(pass);
+                if (pass) {
+                    const int at = staged + __popcll(m & ((1ull << lane) - 1ull));
+                    st_v[wave][at] = v[u];
+                    st_c[wave][at] = (int)c;
+                }
+                staged += __popcll(m);
+            }
+            if (staged > kTopkStage - kTopkLoads * 64) drain();  // uniform: room for a batch
         }
+        drain();
         block_merge<K>(L, k, ws, wc, fs, fc);
         if (nchunk == 1) {
             write_hits(fs, fc, k, row, cols, ld, matches, top_score, top_index, top_matches);
@@ -193,9 +272,19 @@ k_topk_merge(const float *__restrict__ part_s, const int *__restrict__ part_c,
 
 int topk_width(int k) { return k <= 4 ? 4 : (k <= 10 ? 10 : 16); }
 
+// Batches of kTopkChunk columns per row (the most first-pass chunks a row can
+// have: the workspace holds that many partial lists per row).
 int64_t topk_chunks(int64_t cols) {
     const int64_t n = (cols + kTopkChunk - 1) / kTopkChunk;
     return n < 1 ? 1 : n;
+}
+
+// Columns of one first-pass chunk for a rows x cols block: whole batches,
+// about kTopkBlocks chunks in all, at most kTopkMaxBatches batches each.
+int64_t topk_chunk_cols(int64_t rows, int64_t cols) {
+    int64_t m = rows * topk_chunks(cols) / kTopkBlocks;
+    m = m < 1 ? 1 : (m > kTopkMaxBatches ? kTopkMaxBatches : m);
+    return m * kTopkChunk;
 }
 
 // Per-row count of score >= threshold (NaN never passes).
@@ -293,10 +382,10 @@ extern "C" cs_status cs_topk_rows_workspace_size(int64_t rows, int64_t cols, int
 template <int K>
 void launch_topk(const float *score, const int32_t *matches, int64_t rows, int64_t cols,
                  int64_t ld, int k, float *top_score, int32_t *top_index, int32_t *top_matches,
-                 float *part_s, int *part_c, int64_t nchunk, hipStream_t hs) {
+                 float *part_s, int *part_c, int64_t nchunk, int64_t ccols, hipStream_t hs) {
     hipLaunchKernelGGL(k_topk_chunks<K>, dim3(grid_for(rows * nchunk, 1 << 20)),
                        dim3(kTopkThreads), 0, hs, score, matches, rows, cols, ld, k, nchunk,
-                       part_s, part_c, top_score, top_index, top_matches);
+                       ccols, part_s, part_c, top_score, top_index, top_matches);
     if (nchunk > 1)
         hipLaunchKernelGGL(k_topk_merge<K>, dim3(grid_for(rows, 65536)), dim3(kTopkThreads), 0,
                            hs, part_s, part_c, matches, rows, cols, ld, k, nchunk, top_score,
@@ -319,24 +408,28 @@ extern "C" cs_status cs_topk_rows(const float *score, const int32_t *matches, in
     if (need > 0 && (!workspace || workspace_bytes < need))
         return fail(CS_ERR_WORKSPACE, "workspace of %zu bytes < %zu required", workspace_bytes,
                     need);
-    const int64_t nchunk = topk_chunks(cols);
-    const int64_t n = nchunk > 1 ? rows * nchunk * topk_width(k) : 0;
+    // the workspace holds topk_chunks(cols) partial lists per row; a launch
+    // uses nchunk <= that many
+    const int64_t most = topk_chunks(cols);
+    const int64_t n = most > 1 ? rows * most * topk_width(k) : 0;
     Carver c(workspace);
     float *part_s = c.take<float>(n);
     int *part_c = c.take<int>(n);
+    const int64_t ccols = topk_chunk_cols(rows, cols);
+    const int64_t nchunk = (cols + ccols - 1) / ccols < 1 ? 1 : (cols + ccols - 1) / ccols;
     hipStream_t hs = static_cast<hipStream_t>(stream);
     switch (topk_width(k)) {
         case 4:
             launch_topk<4>(score, matches, rows, cols, ld, k, top_score, top_index, top_matches,
-                           part_s, part_c, nchunk, hs);
+                           part_s, part_c, nchunk, ccols, hs);
             break;
         case 10:
             launch_topk<10>(score, matches, rows, cols, ld, k, top_score, top_index, top_matches,
-                            part_s, part_c, nchunk, hs);
+                            part_s, part_c, nchunk, ccols, hs);
             break;
         default:
             launch_topk<16>(score, matches, rows, cols, ld, k, top_score, top_index, top_matches,
-                            part_s, part_c, nchunk, hs);
+                            part_s, part_c, nchunk, ccols, hs);
             break;
     }
     CS_LAUNCH_CHECK("k_topk");

@@ -50,6 +50,49 @@ def test_topk_rows_matches_numpy_order(gpu, rows, cols, k):
     np.testing.assert_array_equal(tm.cpu().numpy(), want_m)
 
 
+@pytest.mark.parametrize("rows,cols,k", [(64, 70000, 10), (240, 140000, 10), (240, 140000, 3)])
+def test_topk_rows_threshold_edge_rows(gpu, rows, cols, k):
+    """Blocks that rank several 4096-column batches each (the first pass
+    inserts only values at or above a per-wave bound), on the rows that
+    bound handles specially: mostly zeros with tied positives, all zeros (the
+    bound is 0 and every value ties), one value everywhere, fewer than k real
+    values among NaN, all NaN, and plain random rows."""
+    import torch
+
+    from cudams.device import topk_rows
+
+    rng = np.random.default_rng(rows + cols + k)
+    s = np.empty((rows, cols), np.float32)
+    for r in range(rows):
+        kind = r % 6
+        if kind == 0:
+            x = (rng.integers(1, 20, cols) / 19.0).astype(np.float32)
+            x[rng.random(cols) < 0.88] = 0.0
+        elif kind == 1:
+            x = np.zeros(cols, np.float32)
+        elif kind == 2:
+            x = np.full(cols, 0.5, np.float32)
+        elif kind == 3:
+            x = np.full(cols, np.nan, np.float32)
+            x[rng.choice(cols, min(k - 1, 2), replace=False)] = rng.random(min(k - 1, 2))
+        elif kind == 4:
+            x = np.full(cols, np.nan, np.float32)
+        else:
+            x = rng.random(cols).astype(np.float32)
+        s[r] = x
+    m = rng.integers(0, 100, size=(rows, cols)).astype(np.int32)
+    S = torch.from_numpy(s).cuda()
+    M = torch.from_numpy(m).cuda()
+    print(f"top-k {rows}x{cols} k={k}", flush=True)
+    ts, ti, tm = topk_rows(S, k, M)
+    torch.cuda.synchronize()
+    ws, wi = ref_topk(s, k)
+    np.testing.assert_array_equal(ti.cpu().numpy(), wi)
+    np.testing.assert_array_equal(ts.cpu().numpy(), ws)
+    want_m = np.where(wi >= 0, m[np.arange(rows)[:, None], np.maximum(wi, 0)], 0)
+    np.testing.assert_array_equal(tm.cpu().numpy(), want_m)
+
+
 def test_topk_fewer_columns_than_k(gpu):
     import torch
 
