@@ -143,23 +143,29 @@ __device__ __forceinline__ float div_short(float n, float d) {
 }
 
 // Measure from the exact counts (float32 integers up to 2^23 bits, so every
-// sum below is exact).  A zero denominator only occurs with inter = 0, where
-// dividing by 1 gives the reference's nan_to_num 0.  Cosine, whose bar is a
-// tolerance (the reference computes it in float32, vector_similarity_
-// functions.py:116), is the count times the hardware reciprocal square root
-// of a * b: within a few float32 ulps of the float64 value.
+// sum below is exact).  The caller passes a and b raised to at least 0.5
+// (count_floor): a count is 0 only for an empty fingerprint, and then inter
+// is 0 too, so the denominator stays positive and the quotient is +0 exactly,
+// the reference's nan_to_num 0, without a per-pair clamp; nonzero counts are
+// unchanged.  Cosine, whose bar is a tolerance (the reference computes it in
+// float32, vector_similarity_functions.py:116), is the count times the
+// hardware reciprocal square root of a * b: within a few float32 ulps of the
+// float64 value.
 template <int kMeasure, bool kShort>
 __device__ __forceinline__ float measure_value(float inter, float a, float b) {
     if (kMeasure == CS_JACCARD) {
-        const float u = fmaxf((a - inter) + b, 1.0f);
+        const float u = (a - inter) + b;
         return kShort ? div_short(inter, u) : div_rn(inter, u);
     }
     if (kMeasure == CS_DICE) {
-        const float u = fmaxf(a + b, 1.0f);
+        const float u = a + b;
         return 2.0f * (kShort ? div_short(inter, u) : div_rn(inter, u));
     }
-    return inter * __builtin_amdgcn_rsqf(fmaxf(a * b, 1.0f));
+    return inter * __builtin_amdgcn_rsqf(a * b);
 }
+
+// A row count for measure_value: the count, or 0.5 for an empty fingerprint.
+__device__ __forceinline__ float count_floor(uint32_t c) { return fmaxf((float)c, 0.5f); }
 
 template <int P>
 __device__ __forceinline__ void plane_mfma(const v4u (&r)[kRF], const v4u (&q)[kQF],
@@ -247,14 +253,14 @@ __device__ __forceinline__ void fp_epilogue(const v4f (&acc)[kQF][kRF], uint32_t
 #pragma unroll
     for (int f = 0; f < kQF; ++f)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) qcount[f][e] = (float)__shfl(pq[f], 4 * grp + e);
+        for (int e = 0; e < 4; ++e) qcount[f][e] = count_floor(__shfl(pq[f], 4 * grp + e));
     const bool flagged = ref_valid || query_valid;
 
 #pragma unroll
     for (int rf = 0; rf < kRF; ++rf) {
         const int64_t r = rb + 16 * rf + lr;
         if (r >= nR) continue;
-        const float a = (float)pr[rf];
+        const float a = count_floor(pr[rf]);
         const bool rv = !ref_valid || ref_valid[r];
 #pragma unroll
         for (int qf = 0; qf < kQF; ++qf) {
@@ -330,13 +336,15 @@ k_fp_gemm(const uint32_t *__restrict__ R, int64_t nR, const uint32_t *__restrict
             const int64_t w1 = (int64_t)(s + 1) * kStepWords + 4 * grp;
             load_step<kFull>(rrow, qrow, w1, n_words, r1, q1);
             fp_step(r0, q0, acc, pr, pq);
-            if (s + 2 < steps) {
-                load_step<kFull>(rrow, qrow, w1 + kStepWords, n_words, r0, q0);
-            } else if (next < n_tiles) {
+            // one load site for the step after (the tile's next, or the next
+            // tile's first): two would make the compiler copy the buffers
+            const bool in_tile = s + 2 < steps;
+            if (!in_tile && next < n_tiles)
                 set_tile(next, gx, wave, lr, R, nR, Q, nQ, n_words, rb_next, qb_next, rrow,
                          qrow);
-                load_step<kFull>(rrow, qrow, 4 * grp, n_words, r0, q0);
-            }
+            if (in_tile || next < n_tiles)
+                load_step<kFull>(rrow, qrow, in_tile ? w1 + kStepWords : 4 * grp, n_words, r0,
+                                 q0);
             fp_step(r1, q1, acc, pr, pq);
         }
         fp_epilogue<kMeasure, kShort>(acc, pr, pq, rb, qb, lr, grp, nR, nQ, ref_valid, query_valid,
