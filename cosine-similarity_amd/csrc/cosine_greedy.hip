@@ -519,12 +519,18 @@ __device__ __forceinline__ void wave_sync() {
 // contributes at most kMidCap entries (more means the pair is settled by the
 // heavy kernel anyway).
 constexpr int kPool = 128;
+// Pool arrays hold kPool entries and a spare (index kPool) that an emit with
+// the pool full writes to instead of branching; pdrop likewise has a spare
+// entry (index kTileQueries).  Nothing reads the spares.  kPool + 2 keeps the
+// arrays after the float64 values 8-byte aligned.
+constexpr int kPoolSlots = kPool + 2;
+constexpr int kDropSlots = kTileQueries + 1;
 constexpr int kIdxRanks = 16;   // >= the largest light_candidates
 constexpr int kMidCap = 64;     // candidates a wave settles for one pair in registers
 constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first
 constexpr size_t kWaveLds =
-    ((size_t)kPool * (8 + 4 + 1) + (size_t)kTileQueries * (4 + 1 + kIdxRanks + 1) + 16 + kWave * 4 +
-     15) / 16 * 16;
+    ((size_t)kPoolSlots * (8 + 4 + 1) + (size_t)kTileQueries * (4 + kIdxRanks + 1) + kDropSlots +
+     16 + kWave * 4 + 15) / 16 * 16;
 constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
 // Slots of the block-relative reference offsets in LDS: kMaxLightRefs + 1
 // rounded up so that the pools after them start 16-byte aligned (their
@@ -627,8 +633,8 @@ __device__ __forceinline__ WavePool wave_pool(unsigned char *lds, int wave) {
     unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kRoffSlots);
     WavePool w;
     w.pv = reinterpret_cast<double *>(pools + wave * kWaveLds);
-    w.pij = reinterpret_cast<uint32_t *>(w.pv + kPool);
-    w.cnt = reinterpret_cast<int *>(w.pij + kPool);
+    w.pij = reinterpret_cast<uint32_t *>(w.pv + kPoolSlots);
+    w.cnt = reinterpret_cast<int *>(w.pij + kPoolSlots);
     w.pool_n = w.cnt + kTileQueries;
     w.midx = w.pool_n + 4;
     w.pq = reinterpret_cast<unsigned char *>(w.midx + kWave);
@@ -710,13 +716,13 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kRoffSlots);
     unsigned char *wbase = pools + wave * kWaveLds;
     double *pv = reinterpret_cast<double *>(wbase);
-    uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPool);
-    int *cnt = reinterpret_cast<int *>(pij + kPool);
+    uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPoolSlots);
+    int *cnt = reinterpret_cast<int *>(pij + kPoolSlots);
     int *pool_n = cnt + kTileQueries;
     int *midx = pool_n + 4;
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
-    unsigned char *pdrop = pq + kPool;  // query lost a candidate to a full pool
-    unsigned char *pidx = pdrop + kTileQueries;  // [kIdxRanks + 1][kTileQueries] pool slots
+    unsigned char *pdrop = pq + kPoolSlots;  // query lost a candidate to a full pool
+    unsigned char *pidx = pdrop + kDropSlots;  // [kIdxRanks + 1][kTileQueries] pool slots
     float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * kWaveLds);
     float *sint_l = smz_l + cap + 1;
     uint16_t *dir_l = reinterpret_cast<uint16_t *>(sint_l + cap + 1);
@@ -865,23 +871,23 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     // rank within the query and pool slot, both atomics in flight at once
                     const int rank = atomicAdd(&cnt[ql], 1);
                     const int pos = atomicAdd(pool_n, 1);
-                    if ((rank >= kMidCap) | (pos >= kPool)) {  // rare
-                        if (rank >= kMidCap) {  // heavy anyway: the slot holds no candidate
-                            if (pos < kPool) pq[pos] = 0xffu;
-                        } else {
-                            pdrop[ql] = 1;
-                        }
-                        return;
-                    }
+                    // The rare cases take no branch.  A pair past kMidCap
+                    // candidates goes to the heavy kernel: its slot is marked
+                    // empty.  With the pool full the entry goes to the spare
+                    // slot and the query is probed again in a later pass
+                    // (pdrop; a dropped pair reads nothing of this pass).
+                    const bool room = pos < kPool, keep = rank < kMidCap;
+                    const int slot = room ? pos : kPool;
                     if constexpr (F32 && TF)  // ((1 * int_pow_r) * 1) * int_pow_q
-                        pv[pos] = rin * (double)qin;
+                        pv[slot] = rin * (double)qin;
                     // otherwise the value needs the query peak's arrays in
                     // HBM: filled for the whole pool after the probe
-                    pij[pos] = pack_ij(i, j);
-                    pq[pos] = (unsigned char)ql;
+                    pij[slot] = pack_ij(i, j);
+                    pq[slot] = keep ? (unsigned char)ql : (unsigned char)0xffu;
                     // ranks past the index go to a spare row nobody reads
                     pidx[(rank < kIdxRanks ? rank : kIdxRanks) * kTileQueries + ql] =
-                        (unsigned char)pos;
+                        (unsigned char)slot;
+                    pdrop[(room | !keep) ? kTileQueries : ql] = 1;
                 };
                 if (in_lds) {
                     // tile_probe on the LDS copy, with the skip done as a
