@@ -145,6 +145,7 @@ def light_roofline(n, lens, light_ms) -> dict:
     if valu is not None:
         rec["limiter"] = "valu_issue"
         rec["valu_frac"] = valu
+        rec["wait_frac"] = profile_field("k_light", f"cosine_{n}", "wave_wait")
     return rec
 
 
@@ -248,7 +249,7 @@ def bench_cosine(a) -> dict:
         log("parity against the oracle")
         res["parity"] = cosine_parity(got, want, packed, tol, ml)
         log(f"parity {res['parity']}")
-    if not a.no_cpu_parallel:
+    if not a.no_cpu_baseline and not a.no_cpu_parallel:
         res["cpu_parallel"] = bench_cpu_parallel(packed, tol, want)
     del got, want
     if not a.no_matrix_wall:
@@ -574,7 +575,8 @@ def main() -> int:
     ap.add_argument("--no-fingerprint", action="store_true",
                     help="skip the fingerprint section of the cosine line")
     ap.add_argument("--no-cpu-parallel", action="store_true",
-                    help="skip the CPUParallelCosineGreedy timing of the cosine line")
+                    help="skip the CPUParallelCosineGreedy timing of the cosine line "
+                         "(--no-cpu-baseline skips it too)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched-jobs section of the cosine line")
     ap.add_argument("--no-serving", action="store_true",
