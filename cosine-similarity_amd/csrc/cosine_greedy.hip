@@ -512,12 +512,12 @@ __device__ __forceinline__ void wave_sync() {
 
 // Per-wave candidate pool of k_light: every exact candidate of the current
 // reference against the tile, in emit order, with its query lane (pq).  The
-// first kIdxRanks candidates of each query are also indexed by their rank
-// (pidx[rank][query], the rank is what the query's count atomic returns; one
-// spare row takes every later rank, so the store needs no branch), so a lane
-// gathers its pair's candidates with independent loads.  A query
-// contributes at most kMidCap entries (more means the pair is settled by the
-// heavy kernel anyway).
+// first K (light_candidates) candidates of each query are also indexed by
+// their rank (pidx[rank][query], the rank is what the query's count atomic
+// returns; one spare row takes every later rank, so the store needs no
+// branch), so a lane gathers its pair's candidates with independent loads.  A
+// query contributes at most kMidCap entries (more means the pair is settled
+// by the heavy kernel anyway).
 constexpr int kPool = 128;
 // Pool arrays hold kPool entries and a spare (index kPool) that an emit with
 // the pool full writes to instead of branching; pdrop likewise has a spare
@@ -525,37 +525,59 @@ constexpr int kPool = 128;
 // arrays after the float64 values 8-byte aligned.
 constexpr int kPoolSlots = kPool + 2;
 constexpr int kDropSlots = kTileQueries + 1;
-constexpr int kIdxRanks = 16;   // >= the largest light_candidates
 constexpr int kMidCap = 64;     // candidates a wave settles for one pair in registers
 constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first
-constexpr size_t kWaveLds =
-    ((size_t)kPoolSlots * (8 + 4 + 1) + (size_t)kTileQueries * (4 + kIdxRanks + 1) + kDropSlots +
-     16 + kWave * 4 + 15) / 16 * 16;
-constexpr int kDirPad = (kTileBuckets + 1 + 7) / 8 * 8;
 // Slots of the block-relative reference offsets in LDS: kMaxLightRefs + 1
 // rounded up so that the pools after them start 16-byte aligned (their
 // float64 values are read and written as 8-byte LDS accesses).
 constexpr int kRoffSlots = (kMaxLightRefs + 1 + 3) / 4 * 4;
 constexpr size_t kPoolBase = (size_t)kTileQueries * (8 + 8) + (size_t)kRoffSlots * 4;
-static_assert(kPoolBase % 16 == 0 && kWaveLds % 16 == 0, "pool values are 8-byte aligned");
-
-// Bytes of dynamic LDS of k_light for a tile capacity `cap`.
-constexpr size_t light_lds_bytes(int cap) {
-    return (size_t)kTileQueries * (8 + 8)          // qoff, qnorm
-           + (size_t)kRoffSlots * 4                 // reference offsets (block-relative)
-           + (size_t)kLightWaves * kWaveLds         // pools
-           + (size_t)(cap + 1) * 4                  // sorted values (f32)
-           + (size_t)(cap + 1) * 4                  // sorted int_pow (f32)
-           + (size_t)kDirPad * 2                    // directory (u16)
-           + (size_t)(cap + 1) * 2                  // peak index (u16)
-           + (size_t)(cap + 1 + 15) / 16 * 16;      // query lane (u8)
-}
 constexpr int kLightPeakBytes = 4 + 4 + 2 + 1;
 // LDS of one gfx950 workgroup (and CU): the lane kernel is laid out for it at
 // compile time, so every LDS array sits at a constant offset.
 constexpr size_t kLdsBytes = 160 * 1024;
-constexpr int kLightCap = (int)((kLdsBytes - light_lds_bytes(0) - 64) / kLightPeakBytes) - 1;
-static_assert(light_lds_bytes(kLightCap) <= kLdsBytes, "lane kernel LDS fits a workgroup");
+
+// LDS layout of the lane kernel for K = light_candidates.  The rank index
+// has K rows and a spare.  The directory over the tile's sorted values has
+// `buckets` buckets: twice the prepared directory's for K <= 8, paid for by
+// the rank rows those variants do not need, so the probe's binary search
+// starts from half as many entries.  A tile of at most `cap` peaks is staged
+// in LDS; larger tiles are probed in HBM through the prepared directory.
+template <int K>
+struct LightLayout {
+    static constexpr int ranks = K;
+    static constexpr int buckets = K <= 8 ? 2 * kTileBuckets : kTileBuckets;
+    static constexpr size_t wave_bytes =
+        ((size_t)kPoolSlots * (8 + 4 + 1) + (size_t)kTileQueries * (4 + ranks + 1) +
+         kDropSlots + 16 + kWave * 4 + 15) / 16 * 16;
+    static constexpr int dir_pad = (buckets + 1 + 7) / 8 * 8;
+    // bytes of dynamic LDS for a tile capacity `c`
+    static constexpr size_t bytes(int c) {
+        return (size_t)kTileQueries * (8 + 8)          // qoff, qnorm
+               + (size_t)kRoffSlots * 4                 // reference offsets (block-relative)
+               + (size_t)kLightWaves * wave_bytes       // pools
+               + (size_t)(c + 1) * 4                    // sorted values (f32)
+               + (size_t)(c + 1) * 4                    // sorted int_pow (f32)
+               + (size_t)dir_pad * 2                    // directory (u16)
+               + (size_t)(c + 1) * 2                    // peak index (u16)
+               + (size_t)(c + 1 + 15) / 16 * 16;        // query lane (u8)
+    }
+    static constexpr int cap = (int)((kLdsBytes - bytes(0) - 64) / kLightPeakBytes) - 1;
+    static_assert(kPoolBase % 16 == 0 && wave_bytes % 16 == 0, "pool values are 8-byte aligned");
+    static_assert(bytes(cap) <= kLdsBytes, "lane kernel LDS fits a workgroup");
+    static_assert(cap < 65535, "directory entries are 16-bit");
+};
+
+// Bucket of x among `nb` equal buckets over the tile's values: nb =
+// kTileBuckets << s uses inv_w << s (exact), so it refines the prepared
+// directory's buckets and stays monotone in x.
+template <int NB>
+__device__ __forceinline__ int light_bucket(float x, float lo, float inv_w_nb) {
+    const float f = (x - lo) * inv_w_nb;
+    if (!(f > 0.0f)) return 0;
+    if (f >= (float)(NB - 1)) return NB - 1;
+    return (int)f;
+}
 
 // Descending bitonic sort of one key per lane across the wave (64 lanes),
 // partner values from xor_lane (cs_internal.h).
@@ -615,7 +637,7 @@ struct RefChunk {
 };
 
 // LDS layout of the lane kernel's per-wave candidate pool (see
-// light_lds_bytes): values, (i, j), per-query counts, pool count, compaction
+// LightLayout): values, (i, j), per-query counts, pool count, compaction
 // slots, query of each slot.
 struct WavePool {
     double *pv;
@@ -626,13 +648,10 @@ struct WavePool {
     unsigned char *pq;
 };
 
-__device__ __forceinline__ WavePool wave_pool(unsigned char *lds, int wave) {
-    int64_t *qoff_l = reinterpret_cast<int64_t *>(lds);
-    double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
-    int32_t *roff_l = reinterpret_cast<int32_t *>(qnorm_l + kTileQueries);
-    unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kRoffSlots);
+// The pool of the wave whose pool starts `pool_off` bytes into the LDS.
+__device__ __forceinline__ WavePool wave_pool(unsigned char *lds, int pool_off) {
     WavePool w;
-    w.pv = reinterpret_cast<double *>(pools + wave * kWaveLds);
+    w.pv = reinterpret_cast<double *>(lds + pool_off);
     w.pij = reinterpret_cast<uint32_t *>(w.pv + kPoolSlots);
     w.cnt = reinterpret_cast<int *>(w.pij + kPoolSlots);
     w.pool_n = w.cnt + kTileQueries;
@@ -651,9 +670,9 @@ struct MidResult {
 // are compacted, their (value, i, j) sorted descending across the lanes, and
 // the greedy pass accepts the first live candidate, then drops every one that
 // shares its i or j.  The same result on every lane.
-__device__ __attribute__((noinline)) MidResult mid_settle(int wave, int lim, int sl) {
+__device__ __attribute__((noinline)) MidResult mid_settle(int pool_off, int lim, int sl) {
     extern __shared__ __align__(16) unsigned char light_lds[];
-    const WavePool P = wave_pool(light_lds, wave);
+    const WavePool P = wave_pool(light_lds, pool_off);
     const int lane = threadIdx.x & (kWave - 1);
     int have = 0;
     for (int base = 0; base < lim; base += kWave) {  // compact the pair's entries
@@ -707,14 +726,16 @@ template <int K, bool F32>
 __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     extern __shared__ __align__(16) unsigned char light_lds[];
     __shared__ int next_ref;
-    constexpr int cap = kLightCap;
+    using LL = LightLayout<K>;
+    constexpr int cap = LL::cap;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    // LDS layout (see light_lds_bytes)
+    // LDS layout (see LightLayout)
     int64_t *qoff_l = reinterpret_cast<int64_t *>(light_lds);
     double *qnorm_l = reinterpret_cast<double *>(qoff_l + kTileQueries);
     int32_t *roff_l = reinterpret_cast<int32_t *>(qnorm_l + kTileQueries);
     unsigned char *pools = reinterpret_cast<unsigned char *>(roff_l + kRoffSlots);
-    unsigned char *wbase = pools + wave * kWaveLds;
+    unsigned char *wbase = pools + wave * LL::wave_bytes;
+    const int pool_off = (int)(wbase - light_lds);
     double *pv = reinterpret_cast<double *>(wbase);
     uint32_t *pij = reinterpret_cast<uint32_t *>(pv + kPoolSlots);
     int *cnt = reinterpret_cast<int *>(pij + kPoolSlots);
@@ -722,11 +743,11 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     int *midx = pool_n + 4;
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
     unsigned char *pdrop = pq + kPoolSlots;  // query lost a candidate to a full pool
-    unsigned char *pidx = pdrop + kDropSlots;  // [kIdxRanks + 1][kTileQueries] pool slots
-    float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * kWaveLds);
+    unsigned char *pidx = pdrop + kDropSlots;  // [K + 1][kTileQueries] pool slots
+    float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * LL::wave_bytes);
     float *sint_l = smz_l + cap + 1;
     uint16_t *dir_l = reinterpret_cast<uint16_t *>(sint_l + cap + 1);
-    uint16_t *wj_l = dir_l + kDirPad;
+    uint16_t *wj_l = dir_l + LL::dir_pad;
     unsigned char *wl_l = reinterpret_cast<unsigned char *>(wj_l + cap + 1);
 
     const int np = a.tile_np[t];
@@ -767,17 +788,6 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                 }
             }
         }
-        int32_t d[(kTileBuckets + 1 + kLightThreads - 1) / kLightThreads];
-#pragma unroll
-        for (int u = 0; u < (kTileBuckets + 1 + kLightThreads - 1) / kLightThreads; ++u) {
-            const int b = tid + u * kLightThreads;
-            d[u] = b <= kTileBuckets ? gdir[b] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < (kTileBuckets + 1 + kLightThreads - 1) / kLightThreads; ++u) {
-            const int b = tid + u * kLightThreads;
-            if (b <= kTileBuckets) dir_l[b] = (uint16_t)d[u];
-        }
     }
     if (tid < kTileQueries) {
         const int64_t q = q0 + tid < a.nQ ? q0 + tid : q0;
@@ -790,6 +800,19 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     if (lane == 0) *pool_n = 0;
     if (tid == 0) next_ref = 0;
     __syncthreads();
+    // The directory of the staged tile, built here from its sorted values:
+    // dir[b] = the first position whose bucket is >= b (b in [0, buckets]).
+    // Position p fills the buckets after its predecessor's, up to its own;
+    // the +inf pad at np fills the rest.
+    const float iwn = iw * (float)(LL::buckets / kTileBuckets);
+    if (in_lds) {
+        for (int p = tid; p <= np; p += kLightThreads) {
+            const int bc = p < np ? light_bucket<LL::buckets>(smz_l[p], lo, iwn) : LL::buckets;
+            const int bp = p > 0 ? light_bucket<LL::buckets>(smz_l[p - 1], lo, iwn) : -1;
+            for (int b = bp + 1; b <= bc; ++b) dir_l[b] = (uint16_t)p;
+        }
+        __syncthreads();
+    }
 
     const int64_t q = q0 + lane;
     const bool valid = q < a.nQ;
@@ -885,7 +908,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     pij[slot] = pack_ij(i, j);
                     pq[slot] = keep ? (unsigned char)ql : (unsigned char)0xffu;
                     // ranks past the index go to a spare row nobody reads
-                    pidx[(rank < kIdxRanks ? rank : kIdxRanks) * kTileQueries + ql] =
+                    pidx[(rank < LL::ranks ? rank : LL::ranks) * kTileQueries + ql] =
                         (unsigned char)slot;
                     pdrop[(room | !keep) ? kTileQueries : ql] = 1;
                 };
@@ -896,7 +919,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     // on hold values above it (bucket() is monotone), so the
                     // first value >= x0 sits in [dir[b], dir[b+1]].  About 3
                     // rounds per wave instead of about 8 for the linear walk.
-                    const int b = tile_bucket(x0, lo, iw);
+                    const int b = light_bucket<LL::buckets>(x0, lo, iwn);
                     int pl = (int)dir_l[b], ph = (int)dir_l[b + 1];
                     while (pl < ph) {
                         const int mid = (pl + ph) >> 1;
@@ -1024,7 +1047,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                 else
                     top = mid - 1;
             }
-            static_assert(K <= kIdxRanks, "every small pair's candidates are indexed by rank");
+            static_assert(K <= LL::ranks, "every small pair's candidates are indexed by rank");
             LaneCandidates<K> cand;
             cand.reset();
 #pragma unroll
@@ -1055,7 +1078,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             while (mid) {
                 const int sl = __builtin_ctzll(mid);
                 mid &= mid - 1ull;
-                const MidResult res = mid_settle(wave, used < kPool ? used : kPool, sl);
+                const MidResult res = mid_settle(pool_off, used < kPool ? used : kPool, sl);
                 if (lane == sl) {
                     out_score = (float)(res.sc / (rnorm * qnorm_l[sl]));
                     out_matches = res.m;
@@ -1981,10 +2004,28 @@ cs_status check_spectra(const cs_spectra *s, const char *name) {
     return CS_OK;
 }
 
-// Tile capacity of k_light (kLightCap), or 0 when the device offers less LDS
-// per workgroup than the kernel's compile-time layout needs.
+// Dynamic LDS bytes of the lane kernel for light_candidates K and tile
+// capacity `cap` (K as the launchers round it: 1, 4, 16, else 8).
+size_t light_lds_bytes(int K, int cap) {
+    switch (K) {
+        case 1: return LightLayout<1>::bytes(cap);
+        case 4: return LightLayout<4>::bytes(cap);
+        case 16: return LightLayout<16>::bytes(cap);
+        default: return LightLayout<8>::bytes(cap);
+    }
+}
+int light_layout_cap(int K) {
+    switch (K) {
+        case 1: return LightLayout<1>::cap;
+        case 4: return LightLayout<4>::cap;
+        case 16: return LightLayout<16>::cap;
+        default: return LightLayout<8>::cap;
+    }
+}
+
+// Tile capacity of k_light for K, or 0 when the device offers less LDS per
+// workgroup than the kernel's compile-time layout needs.
 int light_capacity(int K) {
-    (void)K;
     static int max_lds = -1;
     if (max_lds < 0) {
         int dev = 0, v = 0;
@@ -1993,7 +2034,8 @@ int light_capacity(int K) {
             v = 0;
         max_lds = v;
     }
-    return (size_t)max_lds >= light_lds_bytes(kLightCap) ? kLightCap : 0;
+    const int cap = light_layout_cap(K);
+    return (size_t)max_lds >= light_lds_bytes(K, cap) ? cap : 0;
 }
 
 template <int K, bool F32>
@@ -2001,8 +2043,8 @@ cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
     a.lds_cap = light_capacity(K);
     if (a.lds_cap == 0)
         return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
-                    light_lds_bytes(kLightCap));
-    const size_t lds = light_lds_bytes(a.lds_cap);
+                    LightLayout<K>::bytes(LightLayout<K>::cap));
+    const size_t lds = LightLayout<K>::bytes(a.lds_cap);
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_light<K, F32>),
@@ -2237,7 +2279,7 @@ size_t &light_batch_attr() {
 template <int K, bool F32>
 cs_status prime_light_batch(int cap) {
     return set_lds_attr(reinterpret_cast<const void *>(&k_light_batch<K, F32>),
-                        light_lds_bytes(cap), light_batch_attr<K, F32>());
+                        LightLayout<K>::bytes(cap), light_batch_attr<K, F32>());
 }
 
 template <bool F32>
@@ -2255,7 +2297,7 @@ void launch_light_batch(unsigned grid, hipStream_t st, const LightArgs *jobs,
                         const unsigned long long *items, int64_t n, unsigned long long *next,
                         int cap) {
     hipLaunchKernelGGL((k_light_batch<K, F32>), dim3(grid), dim3(kLightThreads),
-                       light_lds_bytes(cap), st, jobs, items, n, next);
+                       LightLayout<K>::bytes(cap), st, jobs, items, n, next);
 }
 
 template <bool F32>
@@ -2609,7 +2651,7 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
     if (b->cap == 0) {
         delete b;
         return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
-                    light_lds_bytes(kLightCap));
+                    light_lds_bytes(b->K, light_layout_cap(b->K)));
     }
     static size_t heavy_attr_b = 0, giant_attr_b = 0;
     if ((st = b->f32 ? prime_light_batch_k<true>(b->K, b->cap)
