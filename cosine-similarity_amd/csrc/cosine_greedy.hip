@@ -571,11 +571,11 @@ struct LightLayout {
 // Bucket of x among `nb` equal buckets over the tile's values: nb =
 // kTileBuckets << s uses inv_w << s (exact), so it refines the prepared
 // directory's buckets and stays monotone in x.
-template <int NB>
-__device__ __forceinline__ int light_bucket(float x, float lo, float inv_w_nb) {
+constexpr int kMaxLightBuckets = 32768;
+__device__ __forceinline__ int light_bucket(float x, float lo, float inv_w_nb, int nb) {
     const float f = (x - lo) * inv_w_nb;
     if (!(f > 0.0f)) return 0;
-    if (f >= (float)(NB - 1)) return NB - 1;
+    if (f >= (float)(nb - 1)) return nb - 1;
     return (int)f;
 }
 
@@ -744,13 +744,22 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     unsigned char *pq = reinterpret_cast<unsigned char *>(midx + kWave);
     unsigned char *pdrop = pq + kPoolSlots;  // query lost a candidate to a full pool
     unsigned char *pidx = pdrop + kDropSlots;  // [K + 1][kTileQueries] pool slots
-    float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * LL::wave_bytes);
-    float *sint_l = smz_l + cap + 1;
-    uint16_t *dir_l = reinterpret_cast<uint16_t *>(sint_l + cap + 1);
-    uint16_t *wj_l = dir_l + LL::dir_pad;
-    unsigned char *wl_l = reinterpret_cast<unsigned char *>(wj_l + cap + 1);
-
+    // The staged tile is laid out for its own peak count (np + 1 with the +inf
+    // pad), and the directory takes the rest: at least LL::buckets buckets
+    // (what the capacity was computed with), doubled while it fits, so a
+    // typical tile gets a finer one.  The probe computes these addresses from
+    // scalar bases, as it would from constant ones.
     const int np = a.tile_np[t];
+    const int n1 = np + 1;
+    float *smz_l = reinterpret_cast<float *>(pools + kLightWaves * LL::wave_bytes);
+    float *sint_l = smz_l + n1;
+    uint16_t *wj_l = reinterpret_cast<uint16_t *>(sint_l + n1);
+    unsigned char *wl_l = reinterpret_cast<unsigned char *>(wj_l + n1);
+    uint16_t *dir_l = reinterpret_cast<uint16_t *>(wl_l + ((n1 + 1) & ~1));
+    const int dir_room = (int)((int64_t)LL::bytes(cap) -
+                               (int64_t)(reinterpret_cast<unsigned char *>(dir_l) - light_lds)) / 2;
+    int nb = LL::buckets;  // uniform
+    while (2 * nb + 1 <= dir_room && 2 * nb <= kMaxLightBuckets) nb *= 2;
     const int64_t pb = a.tile_pbase[t];
     const float lo = a.tile_lo[t], iw = a.tile_iw[t];
     const float *gsmz = a.smz + pb;
@@ -804,11 +813,11 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     // dir[b] = the first position whose bucket is >= b (b in [0, buckets]).
     // Position p fills the buckets after its predecessor's, up to its own;
     // the +inf pad at np fills the rest.
-    const float iwn = iw * (float)(LL::buckets / kTileBuckets);
+    const float iwn = iw * (float)(nb / kTileBuckets);  // nb / kTileBuckets: a power of two
     if (in_lds) {
         for (int p = tid; p <= np; p += kLightThreads) {
-            const int bc = p < np ? light_bucket<LL::buckets>(smz_l[p], lo, iwn) : LL::buckets;
-            const int bp = p > 0 ? light_bucket<LL::buckets>(smz_l[p - 1], lo, iwn) : -1;
+            const int bc = p < np ? light_bucket(smz_l[p], lo, iwn, nb) : nb;
+            const int bp = p > 0 ? light_bucket(smz_l[p - 1], lo, iwn, nb) : -1;
             for (int b = bp + 1; b <= bc; ++b) dir_l[b] = (uint16_t)p;
         }
         __syncthreads();
@@ -919,7 +928,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                     // on hold values above it (bucket() is monotone), so the
                     // first value >= x0 sits in [dir[b], dir[b+1]].  About 3
                     // rounds per wave instead of about 8 for the linear walk.
-                    const int b = light_bucket<LL::buckets>(x0, lo, iwn);
+                    const int b = light_bucket(x0, lo, iwn, nb);
                     int pl = (int)dir_l[b], ph = (int)dir_l[b + 1];
                     while (pl < ph) {
                         const int mid = (pl + ph) >> 1;
