@@ -12,9 +12,10 @@ references:
 * :class:`SearchServer` -- continuous batching.  A dispatcher thread takes a
   free *slot* (its own stream, call scratch, output block and pinned staging
   buffers, sized by a memory budget), fuses every queued request that fits
-  into it, and launches search + top-k on the slot's stream; a completion
-  thread hands each request its rows when the slot's event fires and frees
-  the slot.  Requests that arrive while batches run are admitted into the
+  into it, and launches search + top-k on the slot's stream; the slot's own
+  completion thread hands each request its rows when the slot's event fires
+  and frees the slot (a small batch is not held behind a larger one launched
+  before it).  Requests that arrive while batches run are admitted into the
   next slot that frees up, and batches on different slots overlap on the GPU;
 * :class:`PlainSearch` -- the baseline: one search + top-k call per request,
   one request at a time, on the same prepared index.
@@ -25,6 +26,7 @@ library index ascending).
 """
 from __future__ import annotations
 
+import gc
 import queue
 import threading
 import time
@@ -247,7 +249,7 @@ class SearchServer:
     request larger than one slot is rejected at submit.
     """
 
-    def __init__(self, index: LibraryIndex, k: int = 10, slots: int = 3,
+    def __init__(self, index: LibraryIndex, k: int = 10, slots: int = 2,
                  memory_budget: int = 6 << 30, max_batch_spectra: int = 512,
                  max_batch_peaks: Optional[int] = None):
         self.ix, self.k = index, int(k)
@@ -262,7 +264,9 @@ class SearchServer:
         for s in self.slots:
             self.free.put(s)
         self.pending: "queue.Queue[Optional[_Request]]" = queue.Queue()
-        self.inflight: "queue.Queue[Optional[_Slot]]" = queue.Queue()
+        # one completion queue per slot: a slot is finished as soon as its own
+        # batch is done, not behind a larger batch launched before it
+        self.launched = {id(s): queue.Queue() for s in self.slots}
         self._carry: Optional[_Request] = None
         self._threads: List[threading.Thread] = []
         self.batches = 0
@@ -337,19 +341,22 @@ class SearchServer:
                 slot.launch(batch)
                 self.batches += 1
                 self.batch_sizes.append(sum(r.n for r in batch))
-                self.inflight.put(slot)
+                self.launched[id(slot)].put(slot)
         except BaseException as e:  # surface in the client threads
             self.error = e
             self._fail_pending(e)
-        self.inflight.put(None)
+        for q in self.launched.values():
+            q.put(None)
 
-    def _complete(self) -> None:
+    def _complete(self, q: "queue.Queue[Optional[_Slot]]") -> None:
+        """Completion of one slot's batches: wait for its event, hand each
+        request its rows, free the slot."""
         try:
             torch.cuda.set_device(self.ix.device)
         except BaseException as e:
             self.error = e
         while True:
-            slot = self.inflight.get()
+            slot = q.get()
             if slot is None:
                 break
             try:
@@ -362,8 +369,9 @@ class SearchServer:
             self.free.put(slot)
 
     def start(self) -> "SearchServer":
-        for fn in (self._dispatch, self._complete):
-            t = threading.Thread(target=fn, daemon=True)
+        workers = [(self._dispatch, ())] + [(self._complete, (q,)) for q in self.launched.values()]
+        for fn, args in workers:
+            t = threading.Thread(target=fn, args=args, daemon=True)
             t.start()
             self._threads.append(t)
         return self
@@ -438,11 +446,31 @@ class _DeviceRequest:
 def run_load(kind: str, index: LibraryIndex, requests: Sequence[_Request], arrivals: np.ndarray,
              k: int = 10, **server_kw) -> dict:
     """Replay a request stream (arrival times in seconds from the start) on
-    ``kind`` = "served" or "plain"; latency = completion - scheduled arrival."""
-    t0 = time.perf_counter() + 0.05
+    ``kind`` = "served" or "plain"; latency = completion - scheduled arrival.
+    The server (slots, streams, pinned staging) or the plain searcher is set
+    up before the stream's clock starts: setting up is not request latency,
+    and allocating a few GB of slots can take longer than the first arrivals'
+    spacing."""
+    # objects that exist before the stream starts are moved out of the
+    # collector's generations, so a full collection during the stream scans
+    # only what the stream itself allocates (a long-lived server does the same
+    # after start-up)
+    gc.collect()
+    gc.freeze()
+    try:
+        return _run_load(kind, index, requests, arrivals, k, **server_kw)
+    finally:
+        gc.unfreeze()
+
+
+def _run_load(kind, index, requests, arrivals, k, **server_kw) -> dict:
     done: List[_Request] = []
+    t_setup = time.perf_counter()
     if kind == "served":
         srv = SearchServer(index, k=k, **server_kw).start()
+        torch.cuda.synchronize(index.device)
+        setup_s = time.perf_counter() - t_setup
+        t0 = time.perf_counter() + 0.005
         try:
             futs = []
             for r, a in zip(requests, arrivals):
@@ -462,6 +490,7 @@ def run_load(kind: str, index: LibraryIndex, requests: Sequence[_Request], arriv
         plain = PlainSearch(index, k)
         q: "queue.Queue[Optional[_Request]]" = queue.Queue()
         err: List[BaseException] = []
+        torch.cuda.synchronize(index.device)
 
         def worker():
             while True:
@@ -477,6 +506,8 @@ def run_load(kind: str, index: LibraryIndex, requests: Sequence[_Request], arriv
 
         th = threading.Thread(target=worker, daemon=True)
         th.start()
+        setup_s = time.perf_counter() - t_setup
+        t0 = time.perf_counter() + 0.005
         for r, a in zip(requests, arrivals):
             now = time.perf_counter()
             if t0 + a > now:
@@ -498,6 +529,9 @@ def run_load(kind: str, index: LibraryIndex, requests: Sequence[_Request], arriv
     return {"mode": kind, "requests": len(done), "query_spectra": nspec,
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else 0.0,
             "p99_ms": float(np.percentile(lat, 99)) if len(lat) else 0.0,
+            "p999_ms": float(np.percentile(lat, 99.9)) if len(lat) else 0.0,
+            "max_ms": float(lat.max()) if len(lat) else 0.0,
             "mean_ms": float(lat.mean()) if len(lat) else 0.0,
+            "setup_ms": setup_s * 1e3,
             "requests_per_s": len(done) / span if span > 0 else 0.0,
             "query_spectra_per_s": nspec / span if span > 0 else 0.0, **extra}
