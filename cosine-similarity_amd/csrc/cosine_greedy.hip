@@ -1491,6 +1491,223 @@ __device__ void sort_desc_heavy(double *bv, uint32_t *bij, int n) {
     }
 }
 
+// ---- 64-bit keys of the staged heavy path ----
+// A staged pair has at most kHeavyStage = 1024 peaks per side, so (i, j) fits
+// in 20 bits.  Its key is the top 44 bits of an order-preserving map of v
+// (the unsigned order of the map is the float64 order of v; -0 counts as +0)
+// followed by i and j.  Keys order as (v desc, i desc, j desc) does, except
+// between distinct values that share the 44-bit prefix: after the sort, the
+// caller compares every adjacent pair with equal prefixes exactly and falls
+// back to the (v, ij) sort if any is out of order.  One 64-bit compare and
+// two selects replace three compares and six selects per compare-exchange,
+// and an entry takes 8 bytes instead of 12.  The padding key 0 lies below
+// every real key.
+constexpr int kKeyIjBits = 20;
+constexpr unsigned long long kKeyPrefix = ~((1ull << kKeyIjBits) - 1ull);
+static_assert(kHeavyStage <= 1024, "i and j of a staged pair fit 10 bits each");
+
+__device__ __forceinline__ unsigned long long heavy_key(double v, int i, int j) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v + 0.0);
+    const unsigned long long ord = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+    return (ord & kKeyPrefix) | ((unsigned long long)i << 10) | (unsigned long long)j;
+}
+__device__ __forceinline__ int key_i(unsigned long long k) { return (int)((k >> 10) & 1023u); }
+__device__ __forceinline__ int key_j(unsigned long long k) { return (int)(k & 1023u); }
+
+// Compare-exchange of two keys (a at the lower index).
+__device__ __forceinline__ void key_ce(unsigned long long &a, unsigned long long &b, bool desc) {
+    const bool sw = desc ? (b > a) : (a > b);
+    const unsigned long long t = a;
+    a = sw ? b : a;
+    b = sw ? t : b;
+}
+
+// heavy_reg_pass on keys.
+template <int E>
+__device__ void key_reg_pass(unsigned long long *kb, int size, bool initial, int live) {
+    const int x0 = threadIdx.x * E;
+    if (x0 >= live) return;
+    unsigned long long k[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) k[e] = kb[hphys(x0 + e)];
+    if (initial) {
+#pragma unroll
+        for (int sz = 2; sz <= E; sz <<= 1) {
+#pragma unroll
+            for (int st = sz >> 1; st > 0; st >>= 1) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if (e & st) continue;
+                    key_ce(k[e], k[e + st], ((x0 + e) & sz) == 0);
+                }
+            }
+        }
+    } else {
+        const bool desc = (x0 & size) == 0;
+#pragma unroll
+        for (int st = E >> 1; st > 0; st >>= 1) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                if (e & st) continue;
+                key_ce(k[e], k[e + st], desc);
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) kb[hphys(x0 + e)] = k[e];
+}
+
+// heavy_lds_stage on keys.
+__device__ __forceinline__ void key_lds_stage(unsigned long long *kb, int size, int stride,
+                                              int pairs) {
+    for (int t = threadIdx.x; t < pairs; t += kHeavyThreads) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const int pl = hphys(lo), ph = hphys(hi);
+        const bool desc = (lo & size) == 0;
+        const unsigned long long a = kb[pl], b = kb[ph];
+        if (desc ? (b > a) : (a > b)) {
+            kb[pl] = b;
+            kb[ph] = a;
+        }
+    }
+}
+
+// heavy_lds_stage2 on keys: an entry is stored only if it moved (real keys
+// are distinct, every padding key is 0).
+__device__ __forceinline__ void key_lds_stage2(unsigned long long *kb, int size, int h,
+                                               int quads) {
+    for (int t = threadIdx.x; t < quads; t += kHeavyThreads) {
+        const int lowb = t & (h - 1);
+        const int x = lowb + 4 * (t - lowb);
+        const bool desc = (x & size) == 0;
+        int p[4];
+        unsigned long long k[4], k0[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            p[u] = hphys(x + u * h);
+            k[u] = k0[u] = kb[p[u]];
+        }
+        key_ce(k[0], k[2], desc);
+        key_ce(k[1], k[3], desc);
+        key_ce(k[0], k[1], desc);
+        key_ce(k[2], k[3], desc);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k[u] != k0[u]) kb[p[u]] = k[u];
+    }
+}
+
+template <int E>
+__device__ void key_sort_e(unsigned long long *kb, int n, int n2) {
+    key_reg_pass<E>(kb, 0, true, ((n + E - 1) / E) * E);
+    __syncthreads();
+    for (int size = 2 * E; size <= n2; size <<= 1) {
+        const int live = ((n + size - 1) / size) * size;
+        const int pairs = (live < n2 ? live : n2) >> 1;
+        int stride = size >> 1;
+        for (; stride >= 2 * E; stride >>= 2) {
+            key_lds_stage2(kb, size, stride >> 1, pairs >> 1);
+            __syncthreads();
+        }
+        if (stride >= E) {
+            key_lds_stage(kb, size, stride, pairs);
+            __syncthreads();
+        }
+        key_reg_pass<E>(kb, size, false, live < n2 ? live : n2);
+        __syncthreads();
+    }
+}
+
+// x from lane (lane ^ D) of the wave, for a 64-bit key.
+template <int D>
+__device__ __forceinline__ unsigned long long xor_lane_key(unsigned long long x) {
+    const unsigned lo = (unsigned)xor_lane<D>((int)(unsigned)x);
+    const unsigned hi = (unsigned)xor_lane<D>((int)(unsigned)(x >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+// wave_heavy_stage on keys.
+template <int EPL, int SIZE, int STRIDE>
+__device__ __forceinline__ void wave_key_stage(unsigned long long (&k)[EPL], int lane) {
+    if constexpr (STRIDE >= EPL) {
+        constexpr int D = STRIDE / EPL;
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+            const int x = EPL * lane + e;
+            const bool desc = (x & SIZE) == 0;
+            const bool lower = (x & STRIDE) == 0;
+            const unsigned long long o = xor_lane_key<D>(k[e]);
+            // the lower entry of a descending block keeps the greater key
+            const bool take = (lower == desc) ? (o > k[e]) : (k[e] > o);
+            k[e] = take ? o : k[e];
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < EPL; ++e) {
+            if (e & STRIDE) continue;
+            key_ce(k[e], k[e + STRIDE], ((EPL * lane + e) & SIZE) == 0);
+        }
+    }
+    if constexpr (STRIDE > 1) wave_key_stage<EPL, SIZE, STRIDE / 2>(k, lane);
+}
+template <int EPL, int SIZE>
+__device__ __forceinline__ void wave_key_sizes(unsigned long long (&k)[EPL], int lane) {
+    wave_key_stage<EPL, SIZE, SIZE / 2>(k, lane);
+    if constexpr (SIZE < kWave * EPL) wave_key_sizes<EPL, SIZE * 2>(k, lane);
+}
+// One wave sorts 64 * EPL keys descending in registers (wave_sort_heavy).
+template <int EPL>
+__device__ __attribute__((noinline)) void wave_sort_keys(unsigned long long *kb, int n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long k[EPL];
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) {
+        const int x = EPL * lane + e;
+        k[e] = x < n ? kb[hphys(x)] : 0ull;
+    }
+    wave_key_sizes<EPL, 2>(k, lane);
+#pragma unroll
+    for (int e = 0; e < EPL; ++e) kb[hphys(EPL * lane + e)] = k[e];
+    wave_sync();
+}
+
+// Sorts the first n keys descending (sort_desc_heavy on keys).  Padded to
+// 128 or 256 keys, wave 0 alone sorts and the other waves return at once;
+// otherwise the whole workgroup sorts and every thread returns after a
+// barrier.
+__device__ void sort_keys(unsigned long long *kb, int n) {
+    const int tid = threadIdx.x;
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    if (n2 >= 2 * kWave && n2 <= 4 * kWave) {  // uniform
+        if (tid < kWave) {
+            if (n2 == 2 * kWave)
+                wave_sort_keys<2>(kb, n);
+            else
+                wave_sort_keys<4>(kb, n);
+        }
+        return;
+    }
+    for (int e = n + tid; e < n2; e += kHeavyThreads) kb[hphys(e)] = 0ull;
+    __syncthreads();
+    switch (n2 / kHeavyThreads) {
+    case 16: key_sort_e<16>(kb, n, n2); return;
+    case 8: key_sort_e<8>(kb, n, n2); return;
+    case 4: key_sort_e<4>(kb, n, n2); return;
+    case 2: key_sort_e<2>(kb, n, n2); return;
+    default: break;
+    }
+    for (int size = 2; size <= n2; size <<= 1) {
+        const int live = ((n + size - 1) / size) * size;
+        const int pairs = (live < n2 ? live : n2) >> 1;
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            key_lds_stage(kb, size, stride, pairs);
+            __syncthreads();
+        }
+    }
+}
+
 // Greedy over sorted entries [0, n), run by one wave: chunks of 64 in order;
 // inside a chunk the first live candidate is accepted and every later one
 // sharing its i or j is dropped, repeatedly, which is exactly the sequential
@@ -1579,14 +1796,63 @@ __device__ void greedy_rounds(const double *bv, const uint32_t *bij, int n, uint
     }
 }
 
+// greedy_rounds over sorted keys: the same rounds on (i, j) decoded from the
+// keys; only the accepted candidates' values are computed (value(i, j), the
+// enumeration's own product), and they are summed in chunk order as there.
+template <class Value>
+__device__ void greedy_keys(const unsigned long long *kb, int n, uint32_t *used_r,
+                            uint32_t *used_q, uint32_t *own_r, uint32_t *own_q, Value value,
+                            double &s, int &m) {
+    const int lane = threadIdx.x & (kWave - 1);
+    for (int base = 0; base < n; base += kWave) {
+        const int idx = base + lane;
+        const bool valid = idx < n;
+        const unsigned long long k = valid ? kb[hphys(idx)] : 0ull;
+        const int i = key_i(k), j = key_j(k);
+        bool alive = (int)valid & (int)!bit_set(used_r, i) & (int)!bit_set(used_q, j);
+        unsigned long long acc = 0ull;
+        while (__ballot(alive)) {
+            if (alive) {
+                atomicMin(&own_r[i], (uint32_t)lane);
+                atomicMin(&own_q[j], (uint32_t)lane);
+            }
+            wave_sync();
+            const uint32_t er = alive ? own_r[i] : (uint32_t)kWave;
+            const uint32_t eq = alive ? own_q[j] : (uint32_t)kWave;
+            const bool take = alive & (er == (uint32_t)lane) & (eq == (uint32_t)lane);
+            const unsigned long long f = __ballot(take);
+            acc |= f;
+            const bool lost = alive & ((((f >> (er & 63u)) | (f >> (eq & 63u))) & 1ull) != 0ull);
+            wave_sync();
+            if (alive) {
+                own_r[i] = (uint32_t)kWave;
+                own_q[j] = (uint32_t)kWave;
+            }
+            wave_sync();
+            alive = alive & !take & !lost;
+        }
+        const bool mine = (acc >> lane) & 1ull;
+        const double v = mine ? value(i, j) : 0.0;
+        for (unsigned long long t = acc; t; t &= t - 1ull) s += lane_double(v, __builtin_ctzll(t));
+        m += __popcll(acc);
+        if (mine) {
+            atomicOr(&used_r[i >> 5], 1u << (i & 31));
+            atomicOr(&used_q[j >> 5], 1u << (j & 31));
+        }
+        wave_sync();
+    }
+}
+
 __device__ void write_error(const HeavyArgs &a, int64_t o) {
     a.score[o] = __builtin_nanf("");
     a.matches[o] = -2;
     a.overflow[o] = 1;
 }
 
-// One pair, whole block: stage both spectra, enumerate every candidate once
-// into LDS, sort, greedy.  More than kHeavyBuf candidates: queued as giant.
+// One pair, whole block, by (v, ij) entries: stage both spectra, enumerate
+// every candidate once into LDS, sort, greedy.  More than kHeavyBuf
+// candidates: queued as giant.  The path of launches whose spectra are not
+// staged (heavy_pair_keys is the staged one).
 __device__ __forceinline__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_t q,
                                            unsigned char *lds, int *bn_s) {
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -1712,10 +1978,155 @@ __device__ __forceinline__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_
     __syncthreads();
 }
 
+// One pair of a staged launch, whole block, by 64-bit keys (heavy_key):
+// stage both spectra, enumerate every candidate once into LDS, sort the keys,
+// check the order of adjacent keys with equal prefixes exactly, greedy.  More
+// than kHeavyBuf candidates, or keys that left two distinct values out of
+// order: queued as giant (k_giant settles any pair exactly, by (v, ij)).
+__device__ __forceinline__ void heavy_pair_keys(const HeavyArgs &a, int64_t r, int64_t q,
+                                                unsigned char *lds, int *bn_s) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const SpecView &R = a.R, &Q = a.Q;
+    const int64_t roff = R.off[r], qoff = Q.off[q];
+    const int nr = (int)(R.off[r + 1] - roff), nq = (int)(Q.off[q + 1] - qoff);
+    const int64_t o = r * a.ld + q;
+    constexpr int S = kHeavyStage;
+    // heavy_pair_exact's layout; the keys lie over its value array, and its
+    // ij array (not used here) holds own_r / own_q for the greedy
+    unsigned long long *kb = reinterpret_cast<unsigned long long *>(lds);
+    double *r_w = reinterpret_cast<double *>(lds) + kHeavyBufPad;
+    double *q_d = r_w + S;
+    uint32_t *own_r = reinterpret_cast<uint32_t *>(q_d + S);
+    uint32_t *own_q = own_r + S;
+    float *r_mz = reinterpret_cast<float *>(own_r + kHeavyBufPad);
+    float *q_ms = r_mz + S;
+    uint32_t *used_r = reinterpret_cast<uint32_t *>(q_ms + S);
+    uint32_t *used_q = used_r + a.r_words;
+    const int nrw = (nr + 31) >> 5, nqw = (nq + 31) >> 5;
+    if (nrw > a.r_words || nqw > a.q_words || nr > S || nq > S) {
+        if (tid == 0) write_error(a, o);  // caller's max_peaks was wrong
+        return;
+    }
+    for (int k = tid; k < nr; k += nt) {
+        r_mz[k] = R.mz[roff + k];
+        r_w[k] = (R.mzp ? R.mzp[roff + k] : 1.0) * R.inp[roff + k];
+        own_r[k] = (uint32_t)kWave;
+    }
+    for (int k = tid; k < nq; k += nt) {
+        q_ms[k] = Q.mz[qoff + k] + a.shift32;
+        q_d[k] = Q.inp[qoff + k];
+        own_q[k] = (uint32_t)kWave;
+    }
+    for (int w = tid; w < nrw; w += nt) used_r[w] = 0u;
+    for (int w = tid; w < nqw; w += nt) used_q[w] = 0u;
+    if (tid == 0) *bn_s = 0;
+    __syncthreads();
+
+    // a candidate's value, as the enumeration computes it
+    auto value = [&](int i, int j) {
+        const double v = r_w[i] * (Q.mzp ? Q.mzp[qoff + j] : 1.0);
+        return v * q_d[j];
+    };
+    const int lane = tid & (kWave - 1);
+    for (int ib = tid - lane; ib < nr; ib += nt) {  // wave-uniform rounds
+        const int i = ib + lane;
+        bool act = i < nr;
+        const float ri = act ? r_mz[i] : 0.0f;
+        int j = nq;
+        if (act) {  // first query peak not above the window
+            int lo = 0, hi = nq;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ri - q_ms[mid] > a.tolc) lo = mid + 1;
+                else hi = mid;
+            }
+            j = lo;
+        }
+        act = act && j < nq;
+        const double mr = act ? (R.mzx ? R.mzx[roff + i] : (double)ri) : 0.0;
+        // lockstep over the lanes' windows: one candidate per lane and step,
+        // one LDS atomic per wave and step
+        while (__ballot(act)) {
+            bool push = false;
+            unsigned long long key = 0ull;
+            if (act) {
+                const float qj = q_ms[j];
+                if (ri - qj < -a.tolc) {
+                    act = false;
+                } else {
+                    const double mq = a.exact_from_stage ? (double)qj : exact_mz(Q, qoff + j);
+                    if (exact_window(mr, mq, a.tol, a.shift)) {
+                        key = heavy_key(value(i, j), i, j);
+                        push = true;
+                    }
+                    ++j;
+                    act = j < nq;
+                }
+            }
+            const unsigned long long mask = __ballot(push);
+            int base = 0;
+            if (lane == 0 && mask) base = atomicAdd(bn_s, __popcll(mask));
+            base = __shfl(base, 0);
+            if (push) {
+                const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
+                if (pos < kHeavyBuf) kb[hphys(pos)] = key;
+            }
+        }
+    }
+    __syncthreads();
+    const int C = *bn_s;
+    auto to_giant = [&]() {
+        if (tid == 0) {
+            const unsigned long long slot = atomicAdd(&a.counters[2], 1ull);
+            if ((int64_t)slot < a.heavy_cap) a.giant_queue[slot] = pair_entry(a.batched, a.job, r, q);
+        }
+        __syncthreads();
+    };
+    if (C > kHeavyBuf) {
+        to_giant();
+        return;
+    }
+    sort_keys(kb, C);
+    // Adjacent keys with equal prefixes, in exact order?  A buffer sorted by
+    // wave 0 alone is checked by wave 0 (the other waves returned at once); a
+    // workgroup sort ended with a barrier.
+    const bool by_wave = C > kWave && C <= 4 * kWave;
+    int bad = 0;
+    if (!by_wave || tid < kWave) {
+        for (int x = tid; x + 1 < C; x += by_wave ? kWave : nt) {
+            const unsigned long long k0 = kb[hphys(x)], k1 = kb[hphys(x + 1)];
+            if (((k0 ^ k1) & kKeyPrefix) == 0ull) {
+                const int i0 = key_i(k0), j0 = key_j(k0), i1 = key_i(k1), j1 = key_j(k1);
+                bad |= (int)key_greater(value(i1, j1), pack_ij(i1, j1), value(i0, j0),
+                                        pack_ij(i0, j0));
+            }
+        }
+    }
+    if (__syncthreads_or(bad)) {  // the sentinel -1 is still in the output
+        to_giant();
+        return;
+    }
+    if (tid < kWave) {
+        double sc = 0.0;
+        int m = 0;
+        greedy_keys(kb, C, used_r, used_q, own_r, own_q, value, sc, m);
+        if (tid == 0) {
+            a.score[o] = (float)(sc / (R.norm[r] * Q.norm[q]));
+            a.matches[o] = m;
+            a.overflow[o] = (uint8_t)(C >= a.match_limit);
+        }
+    }
+    __syncthreads();
+}
+
 // Settles the queued pairs: one workgroup per pair, the largest first.  A
 // batch passes its job table (the queue and counters are shared).  When the
 // queue overflowed, every job's matrix is scanned for the -1 sentinel.
-__device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs) {
+// STAGED launches (both spectra of every pair staged in LDS) sort 64-bit keys
+// (heavy_pair_keys), the others (v, ij) entries (heavy_pair); one path per
+// instantiation keeps each within the register budget.
+template <bool STAGED>
+__device__ __forceinline__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs) {
     extern __shared__ __align__(16) unsigned char heavy_lds[];
     __shared__ int bn_s;
     __shared__ unsigned long long item_s;
@@ -1736,7 +2147,10 @@ __device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
             const unsigned long long e =
                 item < big ? a.heavy_queue[a.heavy_cap - 1 - item] : a.heavy_queue[item - big];
             const PairRef p = pair_of(a.batched, e);
-            heavy_pair(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
+            if constexpr (STAGED)
+                heavy_pair_keys(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
+            else
+                heavy_pair(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
         }
     } else {  // queue overflowed: find the -1 sentinels
         for (int jb = 0; jb < n_jobs; ++jb) {
@@ -1754,7 +2168,10 @@ __device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
                 const int n = nlist;
                 for (int k = 0; k < n; ++k) {
                     const int64_t pp = list[k];
-                    heavy_pair(J, pp / J.nQ, pp % J.nQ, heavy_lds, &bn_s);
+                    if constexpr (STAGED)
+                        heavy_pair_keys(J, pp / J.nQ, pp % J.nQ, heavy_lds, &bn_s);
+                    else
+                        heavy_pair(J, pp / J.nQ, pp % J.nQ, heavy_lds, &bn_s);
                 }
                 __syncthreads();
             }
@@ -1763,14 +2180,16 @@ __device__ void heavy_body(const HeavyArgs &a, const HeavyArgs *jobs, int n_jobs
 }
 
 // Two workgroups per compute unit (the LDS allows two): at most 128 VGPRs.
+template <bool STAGED>
 __global__ void __launch_bounds__(kHeavyThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_heavy(HeavyArgs a) {
-    heavy_body(a, nullptr, 1);
+    heavy_body<STAGED>(a, nullptr, 1);
 }
 
+template <bool STAGED>
 __global__ void __launch_bounds__(kHeavyThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_heavy_batch(const HeavyArgs *__restrict__ jobs, int n_jobs) {
-    heavy_body(jobs[0], jobs, n_jobs);
+    heavy_body<STAGED>(jobs[0], jobs, n_jobs);
 }
 
 // Giant pairs (more than kHeavyBuf candidates): one wave per pair, in rounds.
@@ -2263,10 +2682,14 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
     if (hlds > 160 * 1024)
         return fail(CS_ERR_UNSUPPORTED, "max_peaks %d/%d needs %zu bytes of LDS", Rs.max_peaks,
                     Qs.max_peaks, hlds);
-    static size_t heavy_attr = 0;
-    if ((st = set_lds_attr(reinterpret_cast<const void *>(&k_heavy), hlds, heavy_attr)) != CS_OK)
-        return st;
-    hipLaunchKernelGGL(k_heavy, dim3(kHeavyBlocks), dim3(kHeavyThreads), hlds, hs, a);
+    static size_t heavy_attr[2] = {0, 0};
+    const void *heavy_fn = stage ? reinterpret_cast<const void *>(&k_heavy<true>)
+                                 : reinterpret_cast<const void *>(&k_heavy<false>);
+    if ((st = set_lds_attr(heavy_fn, hlds, heavy_attr[stage])) != CS_OK) return st;
+    if (stage)
+        hipLaunchKernelGGL(k_heavy<true>, dim3(kHeavyBlocks), dim3(kHeavyThreads), hlds, hs, a);
+    else
+        hipLaunchKernelGGL(k_heavy<false>, dim3(kHeavyBlocks), dim3(kHeavyThreads), hlds, hs, a);
     CS_LAUNCH_CHECK("k_heavy");
     const size_t glds = giant_lds_bytes(a.buf_cap, r_words, q_words);
     if (glds > 65536)
@@ -2662,11 +3085,12 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
         return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
                     light_lds_bytes(b->K, light_layout_cap(b->K)));
     }
-    static size_t heavy_attr_b = 0, giant_attr_b = 0;
+    static size_t heavy_attr_b[2] = {0, 0}, giant_attr_b = 0;
+    const void *heavy_fn = b->stage ? reinterpret_cast<const void *>(&k_heavy_batch<true>)
+                                    : reinterpret_cast<const void *>(&k_heavy_batch<false>);
     if ((st = b->f32 ? prime_light_batch_k<true>(b->K, b->cap)
                      : prime_light_batch_k<false>(b->K, b->cap)) != CS_OK ||
-        (st = set_lds_attr(reinterpret_cast<const void *>(&k_heavy_batch), b->hlds,
-                           heavy_attr_b)) != CS_OK ||
+        (st = set_lds_attr(heavy_fn, b->hlds, heavy_attr_b[b->stage ? 1 : 0])) != CS_OK ||
         (st = set_lds_attr(reinterpret_cast<const void *>(&k_giant_batch), b->glds,
                            giant_attr_b)) != CS_OK) {
         delete b;
@@ -2747,8 +3171,12 @@ extern "C" cs_status cs_cosine_batch_run(const cs_cosine_batch *b, void *stream)
         launch_light_batch_k<false>(b->K, grid, hs, light, items, n_items, &L.counters[5], b->cap);
     CS_LAUNCH_CHECK("k_light_batch");
     const int n = (int)b->jobs.size();
-    hipLaunchKernelGGL(k_heavy_batch, dim3(kHeavyBlocks), dim3(kHeavyThreads), b->hlds, hs, heavy,
-                       n);
+    if (b->stage)
+        hipLaunchKernelGGL(k_heavy_batch<true>, dim3(kHeavyBlocks), dim3(kHeavyThreads), b->hlds,
+                           hs, heavy, n);
+    else
+        hipLaunchKernelGGL(k_heavy_batch<false>, dim3(kHeavyBlocks), dim3(kHeavyThreads), b->hlds,
+                           hs, heavy, n);
     CS_LAUNCH_CHECK("k_heavy_batch");
     hipLaunchKernelGGL(k_giant_batch, dim3(kGiantGrid), dim3(kWave), b->glds, hs, heavy, n);
     CS_LAUNCH_CHECK("k_giant_batch");

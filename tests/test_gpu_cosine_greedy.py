@@ -465,6 +465,48 @@ def test_heavy_sort_widths(gpu, oracle):
     assert_parity(res, want, 5000)
 
 
+def heavy_near_tie_gadget(n_pack_r, n_pack_q, seed):
+    """References and queries whose every pair is heavy (a dense pack of
+    n_pack_r x n_pack_q candidates near m/z 500, tolerance 1) and holds a
+    three-candidate path (a0, b0), (a0, b1), (a1, b1) near m/z 100 with values
+    1 + 2^-40, (1 + 2^-40)(1 + 2^-41), 1 + 2^-41.  The sequential greedy
+    takes (a0, b1) first and matches one peak of the path; taken in (i, j)
+    order -- the order of equal 44-bit value prefixes in the heavy kernel's
+    64-bit keys -- it would match two."""
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(seed)
+    refs_mz, refs_in, q_mz, q_in = [], [], [], []
+    for k in n_pack_r:
+        pack = np.sort(500.0 + rng.uniform(0, 0.8, k).astype(np.float32)).astype(np.float64)
+        refs_mz.append(np.concatenate([[100.0, 101.5], pack]))
+        refs_in.append(np.concatenate([[1.0 + 2.0 ** -40, 1.0],
+                                       np.round(rng.uniform(0.1, 0.6, k), 6)]))
+    for k in n_pack_q:
+        pack = np.sort(500.0 + rng.uniform(0, 0.8, k).astype(np.float32)).astype(np.float64)
+        q_mz.append(np.concatenate([[99.5, 100.8], pack]))
+        q_in.append(np.concatenate([[1.0, 1.0 + 2.0 ** -41], np.round(rng.uniform(0.1, 0.6, k), 6)]))
+    return PackedSpectra.from_arrays(refs_mz, refs_in), PackedSpectra.from_arrays(q_mz, q_in)
+
+
+def test_heavy_values_below_key_resolution(gpu, oracle):
+    """Heavy pairs whose greedy depends on candidate values that differ only
+    below the heavy kernel's 44-bit key prefix (heavy_near_tie_gadget): the
+    kernel's exact check of equal-prefix neighbours must settle them in value
+    order.  Pack sizes cover the one-wave sort (<= 256 candidates) and the
+    workgroup sort."""
+    refs, queries = heavy_near_tie_gadget((9, 12, 20, 33), (8, 11, 16, 30), 57)
+    say("heavy values below key resolution")
+    res = device_run(refs, queries, tol=1.0, match_limit=5000)
+    want = oracle_csr(oracle, refs, queries, tolerance=1.0, match_limit=5000)
+    c = want["n_candidates"]
+    assert c.min() > 64 and (c <= 256).any() and (c > 512).any()
+    # the path matches one peak in every pair: the pack's matches plus one
+    packs_r, packs_q = np.diff(refs.offsets) - 2, np.diff(queries.offsets) - 2
+    np.testing.assert_array_equal(want["matches"], np.minimum.outer(packs_r, packs_q) + 1)
+    assert_parity(res, want, 5000)
+
+
 @pytest.mark.parametrize("T", [4, 64, 1024])
 def test_giant_pairs(gpu, oracle, T):
     """Pairs with more candidates than the single-pass buffer (4096): dense
