@@ -2934,7 +2934,9 @@ extern "C" cs_status cs_cosine_greedy_indexed(const cs_spectra *refs, const cs_s
 
 struct cs_cosine_batch {
     std::vector<cs_cosine_job> jobs;
-    int K = kDefaultK, T = kDefaultRound, rpb = 256, f32 = 1, cap = 0;
+    int K = kDefaultK, T = kDefaultRound, rpb = 256, cap = 0;
+    std::vector<char> job_f32;  // the job can take the float32 lane kernel
+    int64_t n_f32_items = 0;    // items[0, n_f32_items) are float32 jobs' items
     unsigned grid = 0;
     int r_words = 1, q_words = 1, stage = 1, buf_cap = 0;
     int64_t ntiles = 0, npeaks = 0;
@@ -2994,7 +2996,7 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
         const bool f = J.queries.mz_exact == nullptr && J.shift == 0.0 &&
                        J.refs.mz_exact == nullptr && J.refs.mz_pow == nullptr &&
                        J.queries.mz_pow == nullptr;
-        b->f32 = b->f32 && f;
+        b->job_f32.push_back(f ? 1 : 0);
         maxr = J.refs.max_peaks > maxr ? J.refs.max_peaks : maxr;
         maxq = J.queries.max_peaks > maxq ? J.queries.max_peaks : maxq;
         const int64_t nt = (J.queries.n_spectra + kTileQueries - 1) / kTileQueries;
@@ -3039,7 +3041,10 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
                     maxq, cp.round_capacity);
     }
     // work items (job, reference block, tile), costliest first: references in
-    // the block x (their mean peaks + one settle per query) x tile share
+    // the block x (their mean peaks + one settle per query) x tile share.  The
+    // float32 jobs' items come first: each kernel variant takes its own jobs
+    // (one launch per variant), so a job with float64 m/z, a shift or an m/z
+    // power does not move the others off the float32 variant.
     std::vector<std::pair<double, unsigned long long>> w;
     for (int j = 0; j < n_jobs; ++j) {
         const cs_cosine_job &J = jobs[j];
@@ -3058,11 +3063,18 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
             }
         }
     }
+    const std::vector<char> &jf = b->job_f32;
     std::stable_sort(w.begin(), w.end(),
-                     [](const std::pair<double, unsigned long long> &x,
-                        const std::pair<double, unsigned long long> &y) { return x.first > y.first; });
+                     [&jf](const std::pair<double, unsigned long long> &x,
+                           const std::pair<double, unsigned long long> &y) {
+                         const int fx = jf[x.second >> 48], fy = jf[y.second >> 48];
+                         return fx != fy ? fx > fy : x.first > y.first;
+                     });
     b->items.reserve(w.size());
-    for (auto &x : w) b->items.push_back(x.second);
+    for (auto &x : w) {
+        b->items.push_back(x.second);
+        b->n_f32_items += jf[x.second >> 48];
+    }
     // workspace: tables, then the index and the scratch of all jobs together
     Carver c(nullptr);
     c.take<PrepJob>(n_jobs);
@@ -3088,8 +3100,9 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
     static size_t heavy_attr_b[2] = {0, 0}, giant_attr_b = 0;
     const void *heavy_fn = b->stage ? reinterpret_cast<const void *>(&k_heavy_batch<true>)
                                     : reinterpret_cast<const void *>(&k_heavy_batch<false>);
-    if ((st = b->f32 ? prime_light_batch_k<true>(b->K, b->cap)
-                     : prime_light_batch_k<false>(b->K, b->cap)) != CS_OK ||
+    const int64_t n_items_all = (int64_t)b->items.size();
+    if ((b->n_f32_items > 0 && (st = prime_light_batch_k<true>(b->K, b->cap)) != CS_OK) ||
+        (b->n_f32_items < n_items_all && (st = prime_light_batch_k<false>(b->K, b->cap)) != CS_OK) ||
         (st = set_lds_attr(heavy_fn, b->hlds, heavy_attr_b[b->stage ? 1 : 0])) != CS_OK ||
         (st = set_lds_attr(reinterpret_cast<const void *>(&k_giant_batch), b->glds,
                            giant_attr_b)) != CS_OK) {
@@ -3164,12 +3177,19 @@ extern "C" cs_status cs_cosine_batch_run(const cs_cosine_batch *b, void *stream)
     }
     const int64_t n_items = (int64_t)b->items.size();
     if (n_items == 0) return CS_OK;
-    const unsigned grid = (unsigned)std::min<int64_t>(b->grid, n_items);
-    if (b->f32)
-        launch_light_batch_k<true>(b->K, grid, hs, light, items, n_items, &L.counters[5], b->cap);
-    else
-        launch_light_batch_k<false>(b->K, grid, hs, light, items, n_items, &L.counters[5], b->cap);
-    CS_LAUNCH_CHECK("k_light_batch");
+    // one persistent launch per lane-kernel variant, each over its own jobs'
+    // items with its own work counter
+    const int64_t nf = b->n_f32_items, nd = n_items - nf;
+    if (nf > 0) {
+        const unsigned grid = (unsigned)std::min<int64_t>(b->grid, nf);
+        launch_light_batch_k<true>(b->K, grid, hs, light, items, nf, &L.counters[5], b->cap);
+        CS_LAUNCH_CHECK("k_light_batch");
+    }
+    if (nd > 0) {
+        const unsigned grid = (unsigned)std::min<int64_t>(b->grid, nd);
+        launch_light_batch_k<false>(b->K, grid, hs, light, items + nf, nd, &L.counters[6], b->cap);
+        CS_LAUNCH_CHECK("k_light_batch");
+    }
     const int n = (int)b->jobs.size();
     if (b->stage)
         hipLaunchKernelGGL(k_heavy_batch<true>, dim3(kHeavyBlocks), dim3(kHeavyThreads), b->hlds,

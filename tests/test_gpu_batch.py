@@ -67,6 +67,43 @@ def test_batch_equals_one_call_per_job(gpu, oracle, mz_power):
         np.testing.assert_array_equal(got[k][2], (o["n_candidates"] >= 2048).astype(np.uint8))
 
 
+def test_mixed_batch_variants(gpu, oracle):
+    """Jobs that can take the float32 lane kernel (mz_power 0, float32 m/z, no
+    shift) mixed with jobs that cannot (mz_power 0.5; float64 m/z): each class
+    runs on its own variant in its own persistent launch.  Every job equals
+    its own call bit for bit, and the smallest of each class the oracle."""
+    from cudams import synthetic
+    from cudams.batch import CosineJob
+    from cudams.spectra import PackedSpectra
+
+    rng = np.random.default_rng(64)
+    jobs = []
+    for k in range(12):
+        R = int(np.exp(rng.uniform(np.log(16), np.log(200))))
+        Q = int(np.exp(rng.uniform(np.log(16), np.log(400))))
+        s = int(rng.integers(1 << 30))
+        refs = synthetic.lognormal_spectra(R, s, cluster_fraction=0.2)
+        ques = synthetic.lognormal_spectra(Q, s + 1, cluster_fraction=0.2)
+        if k % 4 == 3:  # m/z that are not float32 numbers: the float64 path
+            ques = PackedSpectra(ques.mz + 1e-7, ques.intensities, ques.offsets)
+        jobs.append(CosineJob(refs, ques, (0.01, 0.1)[k % 2], 0.0, 0.5 if k % 4 == 1 else 0.0,
+                              1.0, 2048))
+    say(f"mixed batch of {len(jobs)} jobs")
+    got, want = run_both(jobs)
+    for k, (g, w) in enumerate(zip(got, want)):
+        for a, b in zip(g, w):
+            np.testing.assert_array_equal(a, b, err_msg=f"job {k}")
+    size = lambda k: jobs[k].refs.n_spectra * jobs[k].queries.n_spectra
+    for cls in (0, 1, 3):
+        k = min((k for k in range(len(jobs)) if k % 4 == cls), key=size)
+        j = jobs[k]
+        o = oracle.cosine_greedy_matrix((j.refs.mz, j.refs.intensities, j.refs.offsets),
+                                        (j.queries.mz, j.queries.intensities, j.queries.offsets),
+                                        tolerance=j.tolerance, mz_power=j.mz_power)
+        np.testing.assert_array_equal(got[k][1], o["matches"], err_msg=f"job {k}")
+        np.testing.assert_allclose(got[k][0], o["score"], rtol=0, atol=SCORE_ATOL)
+
+
 def test_batch_launch_shapes_agree(gpu):
     """Different work-item sizes, register candidates and grids give the same
     results (the shape only changes who computes a pair)."""
