@@ -62,13 +62,16 @@ class PackedSpectra:
                 mz, it = mz[:n_max_peaks], it[:n_max_peaks]
             mzs.append(mz)
             its.append(it)
-        return cls.from_arrays(mzs, its)
+        return cls._concat(mzs, its)  # already float64 1-D arrays
 
     @classmethod
     def from_arrays(cls, mzs: Iterable[np.ndarray], intensities: Iterable[np.ndarray]) -> "PackedSpectra":
-        mzs = [np.asarray(m, np.float64) for m in mzs]
-        its = [np.asarray(i, np.float64) for i in intensities]
-        lens = np.fromiter((len(m) for m in mzs), dtype=np.int64, count=len(mzs))
+        return cls._concat([np.asarray(m, np.float64) for m in mzs],
+                           [np.asarray(i, np.float64) for i in intensities])
+
+    @classmethod
+    def _concat(cls, mzs: list, its: list) -> "PackedSpectra":
+        lens = np.fromiter(map(len, mzs), dtype=np.int64, count=len(mzs))
         off = np.zeros(len(mzs) + 1, np.int64)
         np.cumsum(lens, out=off[1:])
         mz = np.concatenate(mzs) if len(mzs) else np.zeros(0, np.float64)
@@ -84,12 +87,12 @@ class PackedSpectra:
         if len(self.mz) != len(self.intensities):
             raise ValueError("m/z and intensity arrays differ in length")
         if len(self.mz):
-            if not (np.all(np.isfinite(self.mz)) and np.all(np.isfinite(self.intensities))):
+            if not (np.isfinite(self.mz).all() and np.isfinite(self.intensities).all()):
                 raise ValueError("m/z and intensities must be finite")
-            d = np.diff(self.mz)
-            inner = np.ones(len(d), bool)
-            inner[off[1:-1][(off[1:-1] > 0) & (off[1:-1] < len(self.mz))] - 1] = False
-            if np.any(d[inner] < 0):
+            # a decrease of m/z is allowed only from a spectrum's last peak
+            # to the next spectrum's first
+            down = np.flatnonzero(self.mz[1:] < self.mz[:-1])
+            if down.size and not np.isin(down + 1, off[1:-1]).all():
                 raise ValueError("peaks must be sorted by m/z within each spectrum")
         if self.n_spectra and int(self.lengths.max()) > MAX_PEAKS:
             raise ValueError(f"a spectrum has more than {MAX_PEAKS} peaks; set n_max_peaks")
@@ -102,7 +105,8 @@ def power_factors(p: PackedSpectra, mz_power: float, intensity_power: float):
     GPU kernel computes the same norms at spectrum_similarity_functions.py:82-110).
     mz_pow is None when mz_power == 0 (every factor is exactly 1.0)."""
     mz_pow = None if mz_power == 0 else p.mz ** mz_power
-    int_pow = p.intensities ** intensity_power
+    # x ** 1.0 is x exactly (IEEE pow); skip the pass for the default power
+    int_pow = p.intensities if intensity_power == 1 else p.intensities ** intensity_power
     w = int_pow if mz_pow is None else mz_pow * int_pow
     seg = np.repeat(np.arange(p.n_spectra), p.lengths)
     norm = np.sqrt(np.bincount(seg, weights=w * w, minlength=p.n_spectra)) if p.n_spectra else \
