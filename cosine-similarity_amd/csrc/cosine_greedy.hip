@@ -379,13 +379,45 @@ k_tile_index(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t 
 // sint[p] = float32 of the int_pow of list position p, and whether every
 // value of the tile survives that rounding (then the lane kernel forms
 // candidate values from these copies exactly).
+//
+// With tile_order, the grid's last workgroup instead writes the order in
+// which a scheduled lane-kernel launch takes the tiles: most peaks first (a
+// tile's lane-kernel time follows its peak count), ties by tile index, so
+// the longest work items start first and the last round is made of the
+// shortest.  It ranks (peaks, ~index) keys in LDS.  Scheduled
+// launches have at most kOrderTiles tiles; more tiles leave the order
+// unwritten (it is not read).
+constexpr int kOrderTiles = 2048;
+__device__ void order_tiles(const int32_t *tile_np, int64_t ntiles, uint32_t *order) {
+    __shared__ unsigned long long key[kOrderTiles];
+    if (ntiles > kOrderTiles) return;
+    const int tid = threadIdx.x, nt = blockDim.x, n = (int)ntiles;
+    for (int t = tid; t < n; t += nt)
+        key[t] = ((unsigned long long)(uint32_t)tile_np[t] << 32) |
+                 (unsigned long long)(0xffffffffu - (uint32_t)t);
+    __syncthreads();
+    // a tile's place is the number of greater keys (the keys are distinct);
+    // every lane reads the same key at once, so the loop is LDS broadcasts
+    for (int t = tid; t < n; t += nt) {
+        const unsigned long long k = key[t];
+        int rank = 0;
+        for (int u = 0; u < n; ++u) rank += key[u] > k ? 1 : 0;
+        order[rank] = (uint32_t)t;
+    }
+}
+
 __global__ void __launch_bounds__(1024)
 k_tile_values(PrepJob one, const PrepJob *jobs, const int32_t *tile_job, int64_t ntiles,
               const int32_t *tile_np, const int64_t *tile_pbase, const uint32_t *swho,
-              float *sint, int32_t *tile_flags) {
+              float *sint, int32_t *tile_flags, uint32_t *tile_order) {
     __shared__ int64_t loff[kTileQueries + 1];
     const int tid = threadIdx.x;
-    for (int64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+    const int64_t nblk = tile_order ? gridDim.x - 1 : gridDim.x;
+    if (blockIdx.x == nblk) {  // the extra workgroup of a tile_order launch
+        order_tiles(tile_np, ntiles, tile_order);
+        return;
+    }
+    for (int64_t g = blockIdx.x; g < ntiles; g += nblk) {
         int64_t t;
         const PrepJob &J = prep_job(one, jobs, tile_job, g, t);
         const double *inp = J.inp;
@@ -1142,6 +1174,25 @@ template <int K, bool F32>
 __global__ void __launch_bounds__(kLightThreads) k_light(LightArgs a) {
     light_body<K, F32>(a, (int64_t)blockIdx.x % a.n_ref_blocks,
                        (int64_t)blockIdx.x / a.n_ref_blocks);
+}
+
+// One call, scheduled: a persistent grid takes (tile, reference block) items
+// from a counter, tiles in `order` (most peaks first, order_tiles), so the
+// longest items start first and the last round is made of the shortest.
+template <int K, bool F32>
+__global__ void __launch_bounds__(kLightThreads)
+k_light_sched(LightArgs a, const uint32_t *__restrict__ order, int64_t n_items,
+              unsigned long long *next) {
+    __shared__ unsigned long long item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(next, 1ull);
+        __syncthreads();
+        const int64_t it = uniform64((int64_t)item_s);
+        __syncthreads();
+        if (it >= n_items) break;
+        light_body<K, F32>(a, it % a.n_ref_blocks, (int64_t)order[it / a.n_ref_blocks]);
+        __syncthreads();  // the next item restages the LDS
+    }
 }
 
 // A batch: a persistent grid takes work items (job << 48 | reference block
@@ -2370,6 +2421,7 @@ struct Layout {
     int32_t *dir;
     float *sint;
     int32_t *tile_flags;
+    uint32_t *tile_order;  // tiles by peak count, descending (order_tiles)
     // per-call scratch (LIGHT / HEAVY)
     unsigned long long *counters, *heavy_queue, *giant_queue;
     int64_t ntiles, heavy_cap;
@@ -2405,6 +2457,7 @@ Layout layout_n(void *index_base, void *scratch_base, int64_t ntiles, int64_t np
     L.dir = c.take<int32_t>(T * (kTileBuckets + 1));
     L.sint = c.take<float>(Q.n_peaks + T);
     L.tile_flags = c.take<int32_t>(T);
+    L.tile_order = c.take<uint32_t>(T);
     L.index_bytes = align_up(c.used, 256);
     Carver d(scratch_base);
     L.counters = d.take<unsigned long long>(8);
@@ -2483,6 +2536,55 @@ cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
     hipLaunchKernelGGL((k_light<K, F32>), grid, dim3(kLightThreads), lds, st, a);
     CS_LAUNCH_CHECK("k_light");
     return CS_OK;
+}
+
+// Compute units of the current device (cached per device; 256 if unknown).
+int cu_count() {
+    static int cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cache[dev] > 0) return cache[dev];
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v < 1)
+        v = 256;
+    cache[dev] = v;
+    return v;
+}
+
+template <int K, bool F32>
+cs_status launch_light_sched(hipStream_t st, LightArgs a, const uint32_t *order, int64_t n_items,
+                             unsigned long long *next) {
+    a.lds_cap = light_capacity(K);
+    if (a.lds_cap == 0)
+        return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
+                    LightLayout<K>::bytes(LightLayout<K>::cap));
+    const size_t lds = LightLayout<K>::bytes(a.lds_cap);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_light_sched<K, F32>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return fail(CS_ERR_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+        attr_set = true;
+    }
+    // as many workgroups as the compute units hold at once (the LDS decides)
+    const int64_t per_cu = (int64_t)(160 * 1024 / (lds + 1024)) > 0 ? 160 * 1024 / (lds + 1024) : 1;
+    const int64_t grid = std::min<int64_t>(n_items, per_cu * cu_count());
+    hipLaunchKernelGGL((k_light_sched<K, F32>), dim3((unsigned)grid), dim3(kLightThreads), lds, st,
+                       a, order, n_items, next);
+    CS_LAUNCH_CHECK("k_light_sched");
+    return CS_OK;
+}
+
+template <bool F32>
+cs_status launch_light_sched_k(int K, hipStream_t st, const LightArgs &a, const uint32_t *order,
+                               int64_t n_items, unsigned long long *next) {
+    switch (K) {
+        case 1: return launch_light_sched<1, F32>(st, a, order, n_items, next);
+        case 4: return launch_light_sched<4, F32>(st, a, order, n_items, next);
+        case 16: return launch_light_sched<16, F32>(st, a, order, n_items, next);
+        default: return launch_light_sched<8, F32>(st, a, order, n_items, next);
+    }
 }
 
 template <bool F32>
@@ -2601,7 +2703,7 @@ cs_status set_lds_attr(const void *fn, size_t bytes, size_t &have) {
 
 // The query-tile index (all of L's tiles), for one job or a batch.
 cs_status launch_prepare(const PrepJob &one, const PrepJob *jobs, const int32_t *tile_job,
-                         const Layout &L, hipStream_t hs) {
+                         const Layout &L, hipStream_t hs, bool order) {
     auto blocks = [](int64_t n, int per) {
         int64_t b = (n + per - 1) / per;
         return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
@@ -2625,9 +2727,10 @@ cs_status launch_prepare(const PrepJob &one, const PrepJob *jobs, const int32_t 
                        tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.tile_lo, L.tile_iw, L.smz,
                        L.swho, L.dir);
     CS_LAUNCH_CHECK("k_tile_sort");
-    hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1)), dim3(1024), 0, hs, one, jobs,
-                       tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.swho, L.sint,
-                       L.tile_flags);
+    // one more workgroup writes the tile order (a batch orders its own items)
+    hipLaunchKernelGGL(k_tile_values, dim3(blocks(L.ntiles, 1) + (order ? 1 : 0)), dim3(1024), 0,
+                       hs, one, jobs, tile_job, L.ntiles, L.tile_np, L.tile_pbase, L.swho, L.sint,
+                       L.tile_flags, order ? L.tile_order : nullptr);
     CS_LAUNCH_CHECK("k_tile_values");
     return CS_OK;
 }
@@ -2650,7 +2753,7 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
     if (nQ == 0) return CS_OK;
     if (stages & CS_STAGE_PREPARE) {
         const PrepJob one{Q.mz, Q.off, Q.inp, nQ, shift32, 0, 0};
-        cs_status pst = launch_prepare(one, nullptr, nullptr, L, hs);
+        cs_status pst = launch_prepare(one, nullptr, nullptr, L, hs, true);
         if (pst != CS_OK) return pst;
     }
     if (nR == 0) return CS_OK;
@@ -2667,9 +2770,21 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
         CS_LAUNCH_CHECK("k_fill_u64");
         const LightArgs la = make_light(Rs, Qs, tol, shift, ml, score, matches, overflow, ld, L,
                                         0, nrb, rpb);
-        const dim3 grid((unsigned)(nrb * L.ntiles));
-        st = light_f32(la, Rs) ? launch_light_k<true>(K, grid, hs, la)
-                               : launch_light_k<false>(K, grid, hs, la);
+        // A few rounds of workgroups per compute unit: the items go longest
+        // first from a counter (the last round would otherwise leave compute
+        // units idle while a few long items finish).  Many rounds: one
+        // workgroup per item, in grid order.
+        const int64_t n_items = nrb * L.ntiles;
+        const int64_t cus = cu_count();
+        const bool f32 = light_f32(la, Rs);
+        if (n_items > cus && n_items <= 8 * cus && L.ntiles <= kOrderTiles) {
+            st = f32 ? launch_light_sched_k<true>(K, hs, la, L.tile_order, n_items, &L.counters[5])
+                     : launch_light_sched_k<false>(K, hs, la, L.tile_order, n_items,
+                                                   &L.counters[5]);
+        } else {
+            const dim3 grid((unsigned)n_items);
+            st = f32 ? launch_light_k<true>(K, grid, hs, la) : launch_light_k<false>(K, grid, hs, la);
+        }
         if (st != CS_OK) return st;
     }
     if (!(stages & CS_STAGE_HEAVY)) return CS_OK;
@@ -3172,7 +3287,7 @@ extern "C" cs_status cs_cosine_batch_run(const cs_cosine_batch *b, void *stream)
     const unsigned long long *items =
         reinterpret_cast<const unsigned long long *>(b->ws + b->off_items);
     if (b->ntiles > 0) {
-        cs_status st = launch_prepare(PrepJob{}, prep, tile_job, L, hs);
+        cs_status st = launch_prepare(PrepJob{}, prep, tile_job, L, hs, false);
         if (st != CS_OK) return st;
     }
     const int64_t n_items = (int64_t)b->items.size();
