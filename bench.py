@@ -123,7 +123,7 @@ def cosine_parity(got: dict, want: dict, packed, tol, match_limit) -> dict:
 
 
 def light_roofline(n, lens, light_ms) -> dict:
-    """Roofline of k_light (the LIGHT stage) against HBM.  Algorithmic bytes
+    """Roofline of the lane kernel (the LIGHT stage) against HBM.  Algorithmic bytes
     per launch are the compulsory ones: the 9 output bytes of every pair
     (score f32 + matches i32 + overflow u8) and each spectrum's peaks read
     once (float32 m/z + float32 intensity, 8 B per peak per side).  The
@@ -138,7 +138,8 @@ def light_roofline(n, lens, light_ms) -> dict:
     rec = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": achieved / HBM_PEAK_GBS,
            "traffic": traffic_from_profiles("k_light", f"cosine_{n}"),
-           "kernel": "k_light", "kernel_ms": light_ms, "algorithmic_bytes": compulsory,
+           "kernel": profile_field("k_light", f"cosine_{n}", "kernel_name") or "k_light",
+           "kernel_ms": light_ms, "algorithmic_bytes": compulsory,
            "effective_gbs": streaming / t / 1e9,
            "effective_note": "8 (n_r + n_q) + 9 bytes per pair, every spectrum re-read per pair"}
     valu = profile_field("k_light", f"cosine_{n}", "valu_busy")
