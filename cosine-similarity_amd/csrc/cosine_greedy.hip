@@ -604,6 +604,7 @@ struct LightLayout {
 // kTileBuckets << s uses inv_w << s (exact), so it refines the prepared
 // directory's buckets and stays monotone in x.
 constexpr int kMaxLightBuckets = 32768;
+constexpr int kDirBuildRefs = 512;  // fewer references per workgroup copy the prepared directory
 __device__ __forceinline__ int light_bucket(float x, float lo, float inv_w_nb, int nb) {
     const float f = (x - lo) * inv_w_nb;
     if (!(f > 0.0f)) return 0;
@@ -790,8 +791,17 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     uint16_t *dir_l = reinterpret_cast<uint16_t *>(wl_l + ((n1 + 1) & ~1));
     const int dir_room = (int)((int64_t)LL::bytes(cap) -
                                (int64_t)(reinterpret_cast<unsigned char *>(dir_l) - light_lds)) / 2;
-    int nb = LL::buckets;  // uniform
-    while (2 * nb + 1 <= dir_room && 2 * nb <= kMaxLightBuckets) nb *= 2;
+    // A workgroup with few references copies the prepared directory of the
+    // tile (kTileBuckets buckets, read with the tile) instead of building a
+    // finer one: the build's divergent bucket fills cost about 4.4 us per
+    // workgroup, as much as a few hundred references gain from the finer
+    // directory.
+    const int64_t r_first = rb * a.refs_per_block;
+    const bool copy_dir =
+        (r_first + a.refs_per_block < a.nR ? a.refs_per_block : a.nR - r_first) < kDirBuildRefs;
+    int nb = copy_dir ? kTileBuckets : LL::buckets;  // uniform
+    if (!copy_dir)
+        while (2 * nb + 1 <= dir_room && 2 * nb <= kMaxLightBuckets) nb *= 2;
     const int64_t pb = a.tile_pbase[t];
     const float lo = a.tile_lo[t], iw = a.tile_iw[t];
     const float *gsmz = a.smz + pb;
@@ -807,6 +817,15 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     const int nrefs = (int)(r0 + a.refs_per_block < a.nR ? a.refs_per_block : a.nR - r0);
     const SpecView &R = a.R, &Q = a.Q;
     const int64_t rbase = R.off[r0];
+    constexpr int kDirPer = (kTileBuckets + kLightThreads) / kLightThreads;
+    int32_t dcopy[kDirPer];
+    if (in_lds && copy_dir) {  // in flight with the tile's loads below
+#pragma unroll
+        for (int u = 0; u < kDirPer; ++u) {
+            const int b = tid + u * kLightThreads;
+            dcopy[u] = b <= kTileBuckets ? gdir[b] : 0;
+        }
+    }
     if (in_lds) {  // several independent loads in flight per thread
         for (int e = tid; e <= np; e += 4 * kLightThreads) {
             float x[4], y[4];
@@ -830,6 +849,13 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
             }
         }
     }
+    if (in_lds && copy_dir) {
+#pragma unroll
+        for (int u = 0; u < kDirPer; ++u) {
+            const int b = tid + u * kLightThreads;
+            if (b <= kTileBuckets) dir_l[b] = (uint16_t)dcopy[u];
+        }
+    }
     if (tid < kTileQueries) {
         const int64_t q = q0 + tid < a.nQ ? q0 + tid : q0;
         qoff_l[tid] = Q.off[q];
@@ -846,7 +872,7 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
     // Position p fills the buckets after its predecessor's, up to its own;
     // the +inf pad at np fills the rest.
     const float iwn = iw * (float)(nb / kTileBuckets);  // nb / kTileBuckets: a power of two
-    if (in_lds) {
+    if (in_lds && !copy_dir) {
         for (int p = tid; p <= np; p += kLightThreads) {
             const int bc = p < np ? light_bucket(smz_l[p], lo, iwn, nb) : nb;
             const int bp = p > 0 ? light_bucket(smz_l[p - 1], lo, iwn, nb) : -1;
