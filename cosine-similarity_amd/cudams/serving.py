@@ -145,8 +145,14 @@ class LibraryIndex:
             self.index.numel(), scratch.data_ptr(), scratch.numel(), s))
 
 
+def _a256(x: int) -> int:
+    return (x + 255) & ~255
+
+
 class _Slot:
-    """Buffers of one in-flight fused batch."""
+    """Buffers of one in-flight fused batch.  A batch's request arrays are
+    packed back to back into one pinned staging buffer and uploaded with one
+    copy, and its top-k rows come back with one copy."""
 
     def __init__(self, ix: LibraryIndex, spectra: int, peaks: int, k: int):
         dev = ix.device
@@ -163,22 +169,15 @@ class _Slot:
         _native.check(_native.load().cs_topk_rows_workspace_size(spectra, ix.n, k,
                                                                   ctypes.byref(nb)))
         self.topk_ws.get(nb.value)
-        self.ts = torch.empty((spectra, k), dtype=torch.float32, device=dev)
-        self.ti = torch.empty((spectra, k), dtype=torch.int32, device=dev)
-        self.tm = torch.empty((spectra, k), dtype=torch.int32, device=dev)
-        pin = dict(pin_memory=True)
-        self.h = {
-            "off": torch.empty(spectra + 1, dtype=torch.int64, **pin),
-            "mz": torch.empty(max(peaks, 1), dtype=torch.float32, **pin),
-            "mzx": torch.empty(max(peaks, 1), dtype=torch.float64, **pin),
-            "mzp": torch.empty(max(peaks, 1), dtype=torch.float64, **pin),
-            "inp": torch.empty(max(peaks, 1), dtype=torch.float64, **pin),
-            "norm": torch.empty(max(spectra, 1), dtype=torch.float64, **pin),
-        }
-        self.d = {name: torch.empty_like(t, device=dev) for name, t in self.h.items()}
-        self.rs = torch.empty((spectra, k), dtype=torch.float32, **pin)
-        self.ri = torch.empty((spectra, k), dtype=torch.int32, **pin)
-        self.rm = torch.empty((spectra, k), dtype=torch.int32, **pin)
+        # offsets, m/z (f32), int_pow, norms, exact m/z, m/z powers
+        pk = max(peaks, 1)
+        self.in_bytes = (_a256(8 * (spectra + 1)) + _a256(4 * pk) + 3 * _a256(8 * pk) +
+                         _a256(8 * max(spectra, 1)))
+        self.h_in = torch.empty(self.in_bytes, dtype=torch.uint8, pin_memory=True)
+        self.d_in = torch.empty(self.in_bytes, dtype=torch.uint8, device=dev)
+        # scores, library indices, matches: [3, rows, k] words of the batch's rows
+        self.d_out = torch.empty(3 * spectra * k, dtype=torch.int32, device=dev)
+        self.h_out = torch.empty(3 * spectra * k, dtype=torch.int32, pin_memory=True)
         self.done = torch.cuda.Event()
         self.batch: List[_Request] = []
         self.rows = 0
@@ -189,48 +188,64 @@ class _Slot:
         n = sum(r.n for r in batch)
         P = sum(r.peaks for r in batch)
         assert n <= self.cap and P <= self.peak_cap
-        h = {name: t.numpy() for name, t in self.h.items()}
-        off, o, s = h["off"], 0, 0
+        exact = all(r.exact for r in batch)
+        mzp = self.ix.mz_power != 0.0
+        # this batch's segments of the staging buffer, back to back
+        seg, o = {}, 0
+        for name, size, dt in (("off", 8 * (n + 1), np.int64), ("mz", 4 * P, np.float32),
+                               ("inp", 8 * P, np.float64), ("norm", 8 * n, np.float64),
+                               ("mzx", 0 if exact else 8 * P, np.float64),
+                               ("mzp", 8 * P if mzp else 0, np.float64)):
+            seg[name] = (o, size, dt)
+            o += _a256(size)
+        hb = self.h_in.numpy()
+        h = {name: hb[a:a + size].view(dt) for name, (a, size, dt) in seg.items()}
+        off, po, s = h["off"], 0, 0
         off[0] = 0
-        exact, mzp = True, self.ix.mz_power != 0.0
         for r in batch:
-            off[s + 1:s + r.n + 1] = r.offsets[1:] + o
-            h["mz"][o:o + r.peaks] = r.mz32
-            h["mzx"][o:o + r.peaks] = r.mz64
+            off[s + 1:s + r.n + 1] = r.offsets[1:] + po
+            h["mz"][po:po + r.peaks] = r.mz32
+            if not exact:
+                h["mzx"][po:po + r.peaks] = r.mz64
             if mzp:
-                h["mzp"][o:o + r.peaks] = r.mz_pow
-            h["inp"][o:o + r.peaks] = r.int_pow
+                h["mzp"][po:po + r.peaks] = r.mz_pow
+            h["inp"][po:po + r.peaks] = r.int_pow
             h["norm"][s:s + r.n] = r.norm
-            exact = exact and r.exact
-            o += r.peaks
+            po += r.peaks
             s += r.n
         self.batch, self.rows = batch, n
         st = self.stream
+        base = self.d_in.data_ptr()
+
+        def ptr(name):  # m/z powers and exact m/z are optional
+            a, size, _ = seg[name]
+            return base + a if size or name not in ("mzx", "mzp") else None
+
+        k = self.k
+        ts = self.d_out[:n * k].view(torch.float32).view(n, k)
+        ti = self.d_out[n * k:2 * n * k].view(n, k)
+        tm = self.d_out[2 * n * k:3 * n * k].view(n, k)
         with torch.cuda.stream(st):
-            for name in ("off", "mz", "inp", "norm") + (("mzx",) if not exact else ()) + \
-                    (("mzp",) if mzp else ()):
-                cnt = {"off": n + 1, "norm": n}.get(name, P)
-                if cnt:
-                    self.d[name][:cnt].copy_(self.h[name][:cnt], non_blocking=True)
+            self.d_in[:o].copy_(self.h_in[:o], non_blocking=True)
             refs = _native.CsSpectra(
-                n, P, self.d["off"].data_ptr(), self.d["mz"].data_ptr(),
-                None if exact else self.d["mzx"].data_ptr(),
-                self.d["mzp"].data_ptr() if mzp else None, self.d["inp"].data_ptr(),
-                self.d["norm"].data_ptr(), max((r.max_peaks for r in batch), default=0),
+                n, P, ptr("off"), ptr("mz"), ptr("mzx"), ptr("mzp"), ptr("inp"), ptr("norm"),
+                max((r.max_peaks for r in batch), default=0),
                 _f32_up(max((r.max_abs_mz for r in batch), default=0.0)))
             self.ix.search(refs, n, self.score, self.matches, self.overflow, self.scratch, st)
-            topk_rows(self.score, self.k, self.matches, rows=n, cols=self.ix.n, stream=st,
-                      workspace=self.topk_ws, out=(self.ts, self.ti, self.tm))
-            self.rs[:n].copy_(self.ts[:n], non_blocking=True)
-            self.ri[:n].copy_(self.ti[:n], non_blocking=True)
-            self.rm[:n].copy_(self.tm[:n], non_blocking=True)
+            topk_rows(self.score, k, self.matches, rows=n, cols=self.ix.n, stream=st,
+                      workspace=self.topk_ws, out=(ts, ti, tm))
+            self.h_out[:3 * n * k].copy_(self.d_out[:3 * n * k], non_blocking=True)
             self.done.record(st)
 
     def finish(self) -> None:
         """Wait for the batch, then hand every request its rows."""
         self.done.synchronize()
         t = time.perf_counter()
-        rs, ri, rm = self.rs.numpy(), self.ri.numpy(), self.rm.numpy()
+        n, k = self.rows, self.k
+        out = self.h_out.numpy()
+        rs = out[:n * k].view(np.float32).reshape(n, k)
+        ri = out[n * k:2 * n * k].reshape(n, k)
+        rm = out[2 * n * k:3 * n * k].reshape(n, k)
         s = 0
         for r in self.batch:
             hits = SearchHits(rs[s:s + r.n].copy(), ri[s:s + r.n].copy(), rm[s:s + r.n].copy())
