@@ -559,6 +559,20 @@ constexpr int kPoolSlots = kPool + 2;
 constexpr int kDropSlots = kTileQueries + 1;
 constexpr int kMidCap = 64;     // candidates a wave settles for one pair in registers
 constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first
+// Heavy pairs of at most kWaveKeys candidates between spectra of at most
+// kWavePeaks peaks are settled by one wave each (eight pairs per heavy
+// workgroup at once) instead of by a whole workgroup.
+constexpr int kWavePeaks = 256;
+constexpr int kWaveKeys = 256;
+// Counters of one call (zeroed by k_fill_u64 before the lane kernel):
+// [0] heavy pairs, [1] next heavy item, [2] giant pairs, [3] next giant
+// item, [4] big heavy pairs, [5] [6] lane-kernel work items (scheduled /
+// batch launches), [7] small heavy pairs, [8] next small pair.  The heavy
+// queue's lower half holds the heavy pairs from its bottom up and the big
+// ones from its middle down; its upper half holds the small ones.
+constexpr int kCounters = 16;
+constexpr int kSmallCount = 7;
+constexpr int kSmallNext = 8;
 // Slots of the block-relative reference offsets in LDS: kMaxLightRefs + 1
 // rounded up so that the pools after them start 16-byte aligned (their
 // float64 values are read and written as 8-byte LDS accesses).
@@ -1052,12 +1066,17 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
         };
         auto queue_heavy = [&](int n) {
             const unsigned long long e = pair_entry(a.batched, a.job, r, q);
-            if (n > kBigPair) {  // from the top, taken first
+            const int64_t half = a.heavy_cap >> 1;
+            if (n <= kWaveKeys && nr <= kWavePeaks &&
+                Q.off[q + 1] - Q.off[q] <= kWavePeaks) {  // one wave per pair
+                const unsigned long long slot = atomicAdd(&a.counters[kSmallCount], 1ull);
+                if ((int64_t)slot < a.heavy_cap - half) a.heavy_queue[half + slot] = e;
+            } else if (n > kBigPair) {  // from the middle down, taken first
                 const unsigned long long slot = atomicAdd(&a.counters[4], 1ull);
-                if ((int64_t)slot < a.heavy_cap) a.heavy_queue[a.heavy_cap - 1 - slot] = e;
+                if ((int64_t)slot < half) a.heavy_queue[half - 1 - slot] = e;
             } else {
                 const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
-                if ((int64_t)slot < a.heavy_cap) a.heavy_queue[slot] = e;
+                if ((int64_t)slot < half) a.heavy_queue[slot] = e;
             }
             out_score = 0.0f;
             out_matches = -1;
@@ -1284,6 +1303,12 @@ struct HeavyArgs {
 
 __device__ __forceinline__ bool bit_set(const uint32_t *b, int x) {
     return (b[x >> 5] >> (x & 31)) & 1u;
+}
+
+// Queues pair (r, q) of `a` for k_giant (called by one thread).
+__device__ __forceinline__ void queue_giant(const HeavyArgs &a, int64_t r, int64_t q) {
+    const unsigned long long slot = atomicAdd(&a.counters[2], 1ull);
+    if ((int64_t)slot < a.heavy_cap) a.giant_queue[slot] = pair_entry(a.batched, a.job, r, q);
 }
 
 // Pad [n, next_pow2(n)) with -inf keys and sort the first next_pow2(n)
@@ -2025,10 +2050,7 @@ __device__ __forceinline__ void heavy_pair(const HeavyArgs &a, int64_t r, int64_
     __syncthreads();
     const int C = *bn_s;
     if (C > kHeavyBuf) {
-        if (tid == 0) {
-            const unsigned long long slot = atomicAdd(&a.counters[2], 1ull);
-            if ((int64_t)slot < a.heavy_cap) a.giant_queue[slot] = pair_entry(a.batched, a.job, r, q);
-        }
+        if (tid == 0) queue_giant(a, r, q);
         __syncthreads();
         return;
     }
@@ -2153,10 +2175,7 @@ __device__ __forceinline__ void heavy_pair_keys(const HeavyArgs &a, int64_t r, i
     __syncthreads();
     const int C = *bn_s;
     auto to_giant = [&]() {
-        if (tid == 0) {
-            const unsigned long long slot = atomicAdd(&a.counters[2], 1ull);
-            if ((int64_t)slot < a.heavy_cap) a.giant_queue[slot] = pair_entry(a.batched, a.job, r, q);
-        }
+        if (tid == 0) queue_giant(a, r, q);
         __syncthreads();
     };
     if (C > kHeavyBuf) {
@@ -2196,6 +2215,131 @@ __device__ __forceinline__ void heavy_pair_keys(const HeavyArgs &a, int64_t r, i
     __syncthreads();
 }
 
+// Bytes of LDS one wave uses for a small pair (heavy_pair_wave): keys at
+// hphys positions, weighted intensities of both spectra, their m/z (later
+// the greedy's owner slots), and the used bitsets.
+constexpr int kWaveKeySlots = (kWaveKeys + kWaveKeys / 16 + 15) / 16 * 16;
+constexpr size_t kWavePairBytes = (size_t)kWaveKeySlots * 8 + (size_t)kWavePeaks * (8 + 8 + 4 + 4) +
+                                  2 * (size_t)(kWavePeaks / 32) * 4;
+static_assert(kHeavyThreads / kWave * kWavePairBytes <= (size_t)(kHeavyBufPad + 2 * kHeavyStage) * 12,
+              "every wave of a staged heavy workgroup has room for a small pair");
+static_assert(kWaveKeys <= 4 * kWave, "a small pair's keys are sorted by one wave");
+
+// One small pair (at most kWaveKeys candidates, spectra of at most kWavePeaks
+// peaks) settled by one wave in its own LDS slice: the steps of
+// heavy_pair_keys (staging, enumeration, key sort, exact check of equal
+// prefixes, greedy) with wave-local synchronisation only, so the result is
+// the same bit for bit.  Returns false, with the output untouched, for a pair
+// that does not fit (the caller queues it as giant).  Every early return
+// is on a value moved to a scalar register: the wave leaves together, and
+// the caller's loop stays uniform.
+__device__ bool heavy_pair_wave(const HeavyArgs &a, int64_t r, int64_t q, unsigned char *slice) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const SpecView &R = a.R, &Q = a.Q;
+    const int64_t roff = uniform64(R.off[r]), qoff = uniform64(Q.off[q]);
+    const int nr = uniform((int)(R.off[r + 1] - roff)), nq = uniform((int)(Q.off[q + 1] - qoff));
+    const int64_t o = r * a.ld + q;
+    unsigned long long *kb = reinterpret_cast<unsigned long long *>(slice);
+    double *r_w = reinterpret_cast<double *>(kb + kWaveKeySlots);
+    double *q_d = r_w + kWavePeaks;
+    float *r_mz = reinterpret_cast<float *>(q_d + kWavePeaks);
+    float *q_ms = r_mz + kWavePeaks;
+    uint32_t *own_r = reinterpret_cast<uint32_t *>(r_mz);  // after the enumeration
+    uint32_t *own_q = reinterpret_cast<uint32_t *>(q_ms);
+    uint32_t *used_r = reinterpret_cast<uint32_t *>(q_ms + kWavePeaks);
+    uint32_t *used_q = used_r + kWavePeaks / 32;
+    if (nr > kWavePeaks || nq > kWavePeaks) return false;
+    for (int k = lane; k < nr; k += kWave) {
+        r_mz[k] = R.mz[roff + k];
+        r_w[k] = (R.mzp ? R.mzp[roff + k] : 1.0) * R.inp[roff + k];
+    }
+    for (int k = lane; k < nq; k += kWave) {
+        q_ms[k] = Q.mz[qoff + k] + a.shift32;
+        q_d[k] = Q.inp[qoff + k];
+    }
+    if (lane < kWavePeaks / 32) {
+        used_r[lane] = 0u;
+        used_q[lane] = 0u;
+    }
+    wave_sync();
+
+    auto value = [&](int i, int j) {  // as heavy_pair_keys computes it
+        const double v = r_w[i] * (Q.mzp ? Q.mzp[qoff + j] : 1.0);
+        return v * q_d[j];
+    };
+    int C = 0;  // uniform
+    for (int ib = 0; ib < nr; ib += kWave) {
+        const int i = ib + lane;
+        bool act = i < nr;
+        const float ri = act ? r_mz[i] : 0.0f;
+        int j = nq;
+        if (act) {  // first query peak not above the window
+            int lo = 0, hi = nq;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (ri - q_ms[mid] > a.tolc) lo = mid + 1;
+                else hi = mid;
+            }
+            j = lo;
+        }
+        act = act && j < nq;
+        const double mr = act ? (R.mzx ? R.mzx[roff + i] : (double)ri) : 0.0;
+        while (__ballot(act)) {
+            bool push = false;
+            unsigned long long key = 0ull;
+            if (act) {
+                const float qj = q_ms[j];
+                if (ri - qj < -a.tolc) {
+                    act = false;
+                } else {
+                    const double mq = a.exact_from_stage ? (double)qj : exact_mz(Q, qoff + j);
+                    if (exact_window(mr, mq, a.tol, a.shift)) {
+                        key = heavy_key(value(i, j), i, j);
+                        push = true;
+                    }
+                    ++j;
+                    act = j < nq;
+                }
+            }
+            const unsigned long long mask = __ballot(push);
+            if (push) {
+                const int pos = C + __popcll(mask & ((1ull << lane) - 1ull));
+                if (pos < kWaveKeys) kb[hphys(pos)] = key;
+            }
+            C += __popcll(mask);
+        }
+    }
+    wave_sync();
+    C = uniform(C);
+    if (C > kWaveKeys) return false;
+    if (C <= 2 * kWave)
+        wave_sort_keys<2>(kb, C);
+    else
+        wave_sort_keys<4>(kb, C);
+    int bad = 0;
+    for (int x = lane; x + 1 < C; x += kWave) {
+        const unsigned long long k0 = kb[hphys(x)], k1 = kb[hphys(x + 1)];
+        if (((k0 ^ k1) & kKeyPrefix) == 0ull) {
+            const int i0 = key_i(k0), j0 = key_j(k0), i1 = key_i(k1), j1 = key_j(k1);
+            bad |= (int)key_greater(value(i1, j1), pack_ij(i1, j1), value(i0, j0), pack_ij(i0, j0));
+        }
+    }
+    if (__ballot(bad != 0)) return false;  // the sentinel -1 is still in the output
+    for (int k = lane; k < nr; k += kWave) own_r[k] = (uint32_t)kWave;
+    for (int k = lane; k < nq; k += kWave) own_q[k] = (uint32_t)kWave;
+    wave_sync();
+    double sc = 0.0;
+    int m = 0;
+    greedy_keys(kb, C, used_r, used_q, own_r, own_q, value, sc, m);
+    if (lane == 0) {
+        a.score[o] = (float)(sc / (R.norm[r] * Q.norm[q]));
+        a.matches[o] = m;
+        a.overflow[o] = (uint8_t)(C >= a.match_limit);
+    }
+    wave_sync();
+    return true;
+}
+
 // Settles the queued pairs: one workgroup per pair, the largest first.  A
 // batch passes its job table (the queue and counters are shared).  When the
 // queue overflowed, every job's matrix is scanned for the -1 sentinel.
@@ -2209,12 +2353,41 @@ __device__ __forceinline__ void heavy_body(const HeavyArgs &a, const HeavyArgs *
     __shared__ unsigned long long item_s;
     __shared__ int nlist;
     __shared__ long long list[kScanChunk];
-    // counters[4] counts the big pairs, queued from the top of the queue
-    // downwards; they are taken first so they do not form the tail
-    const unsigned long long count = a.counters[0], big = a.counters[4];
-    if (count + big == 0) return;
+    // counters[4] counts the big pairs, queued from the middle of the queue
+    // downwards; they are taken first among the workgroup pairs so they do
+    // not form the tail.  The small pairs (the upper half) are settled one
+    // per wave before them.
+    const unsigned long long count = a.counters[0], big = a.counters[4],
+                             small = a.counters[kSmallCount];
+    if (count + big + small == 0) return;
     const int tid = threadIdx.x;
-    if ((int64_t)(count + big) <= a.heavy_cap) {
+    const int64_t half = a.heavy_cap >> 1;
+    if ((int64_t)(count + big) <= half && (int64_t)small <= a.heavy_cap - half) {
+        if constexpr (STAGED) {
+            // The small pairs first, one per wave: every resident wave takes
+            // at most a few, so they form no tail.  One claim per workgroup
+            // for all its waves (one atomic on the shared counter, not one
+            // per wave).  The wave's item and pair are in scalar registers,
+            // so its branches are uniform and it runs each pair whole.
+            const int wave = uniform(tid >> 6);
+            unsigned char *slice = heavy_lds + wave * kWavePairBytes;
+            constexpr int kWaves = kHeavyThreads / kWave;
+            for (;;) {
+                if (tid == 0) item_s = atomicAdd(&a.counters[kSmallNext], (unsigned long long)kWaves);
+                __syncthreads();
+                const unsigned long long base = (unsigned long long)uniform64((int64_t)item_s);
+                __syncthreads();
+                if (base >= small) break;  // the slices become the workgroup's buffers
+                const unsigned long long item = base + (unsigned long long)wave;
+                if (item < small) {
+                    const PairRef p =
+                        pair_of(a.batched, uniform64((int64_t)a.heavy_queue[half + item]));
+                    const HeavyArgs &J = jobs ? jobs[p.job] : a;
+                    if (!heavy_pair_wave(J, p.r, p.q, slice) && (tid & (kWave - 1)) == 0)
+                        queue_giant(J, p.r, p.q);
+                }
+            }
+        }
         for (;;) {
             if (tid == 0) item_s = atomicAdd(&a.counters[1], 1ull);
             __syncthreads();
@@ -2222,12 +2395,23 @@ __device__ __forceinline__ void heavy_body(const HeavyArgs &a, const HeavyArgs *
             __syncthreads();
             if (item >= count + big) break;
             const unsigned long long e =
-                item < big ? a.heavy_queue[a.heavy_cap - 1 - item] : a.heavy_queue[item - big];
+                item < big ? a.heavy_queue[half - 1 - item] : a.heavy_queue[item - big];
             const PairRef p = pair_of(a.batched, e);
             if constexpr (STAGED)
                 heavy_pair_keys(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
             else
                 heavy_pair(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
+        }
+        if constexpr (!STAGED) {  // spectra too large to stage: a workgroup per small pair
+            for (;;) {
+                if (tid == 0) item_s = atomicAdd(&a.counters[kSmallNext], 1ull);
+                __syncthreads();
+                const unsigned long long item = (unsigned long long)uniform64((int64_t)item_s);
+                __syncthreads();
+                if (item >= small) break;
+                const PairRef p = pair_of(a.batched, a.heavy_queue[half + item]);
+                heavy_pair(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
+            }
         }
     } else {  // queue overflowed: find the -1 sentinels
         for (int jb = 0; jb < n_jobs; ++jb) {
@@ -2486,7 +2670,7 @@ Layout layout_n(void *index_base, void *scratch_base, int64_t ntiles, int64_t np
     L.tile_order = c.take<uint32_t>(T);
     L.index_bytes = align_up(c.used, 256);
     Carver d(scratch_base);
-    L.counters = d.take<unsigned long long>(8);
+    L.counters = d.take<unsigned long long>(kCounters);
     L.heavy_queue = d.take<unsigned long long>(L.heavy_cap);
     L.giant_queue = d.take<unsigned long long>(L.heavy_cap);
     L.scratch_bytes = align_up(d.used, 256);
@@ -2792,7 +2976,7 @@ cs_status cosine_greedy_run(const cs_spectra *refs, const cs_spectra *queries,
     const int64_t ml = params->match_limit;
     if (stages & CS_STAGE_LIGHT) {
         // the heavy-pair queue of this call starts empty
-        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)8);
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)kCounters);
         CS_LAUNCH_CHECK("k_fill_u64");
         const LightArgs la = make_light(Rs, Qs, tol, shift, ml, score, matches, overflow, ld, L,
                                         0, nrb, rpb);
@@ -3304,7 +3488,7 @@ extern "C" cs_status cs_cosine_batch_run(const cs_cosine_batch *b, void *stream)
     if (!b->ws) return fail(CS_ERR_INVALID, "batch is not uploaded");
     hipStream_t hs = static_cast<hipStream_t>(stream);
     const Layout &L = b->L;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)8);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, hs, L.counters, (int64_t)kCounters);
     CS_LAUNCH_CHECK("k_fill_u64");
     const PrepJob *prep = reinterpret_cast<const PrepJob *>(b->ws);
     const int32_t *tile_job = reinterpret_cast<const int32_t *>(b->ws + b->off_tile_job);

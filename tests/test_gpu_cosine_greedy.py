@@ -529,16 +529,18 @@ def test_giant_pairs(gpu, oracle, T):
     assert_parity(res, want, 5000)
 
 
-def test_heavy_queue_overflow(gpu):
-    """More heavy pairs than the 2^20-entry queue holds: 1040 spectra whose 70
-    peaks all sit at the same m/z (1 Da apart, tol 0.1), so every pair has
-    exactly the 70 diagonal candidates (above the 64 the lane kernel settles).
-    The overflowing queue is abandoned for a scan of the matches plane; the
-    closed form is matches = 70 and score = cosine of the intensity vectors."""
+@pytest.mark.parametrize("n,k", [(1040, 70), (730, 270)])
+def test_heavy_queue_overflow(gpu, n, k):
+    """More heavy pairs than their half of the 2^20-entry queue holds: n
+    spectra whose k peaks all sit at the same m/z (1 Da apart, tol 0.1), so
+    every pair has exactly the k diagonal candidates (above the 64 the lane
+    kernel settles).  k = 70: pairs settled one per wave (at most 256
+    candidates and peaks), k = 270: pairs settled by a workgroup each.  The
+    overflowing queue is abandoned for a scan of the matches plane; the
+    closed form is matches = k and score = cosine of the intensity vectors."""
     from cudams.spectra import PackedSpectra
 
-    n, k = 1040, 70
-    assert n * n > (1 << 20)
+    assert n * n > (1 << 19)
     rng = np.random.default_rng(21)
     mz = 100.0 + np.arange(k, dtype=np.float64)
     its = rng.random((n, k)).astype(np.float32).astype(np.float64) + 0.05
