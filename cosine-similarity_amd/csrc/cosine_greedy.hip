@@ -2714,15 +2714,19 @@ int light_layout_cap(int K) {
     }
 }
 
-// Tile capacity of k_light for K, or 0 when the device offers less LDS per
-// workgroup than the kernel's compile-time layout needs.
+constexpr const char *kDeviceQueryFailed =
+    "HIP device query failed: no usable device for the HIP runtime this library runs on";
+
+// Tile capacity of k_light for K, 0 when the device offers less LDS per
+// workgroup than the kernel's compile-time layout needs, or -1 when the HIP
+// device query itself fails.
 int light_capacity(int K) {
-    static int max_lds = -1;
+    static int max_lds = -1;  // cached once a query succeeds
     if (max_lds < 0) {
         int dev = 0, v = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            v = 0;
+            return -1;
         max_lds = v;
     }
     const int cap = light_layout_cap(K);
@@ -2732,6 +2736,7 @@ int light_capacity(int K) {
 template <int K, bool F32>
 cs_status launch_light(dim3 grid, hipStream_t st, LightArgs a) {
     a.lds_cap = light_capacity(K);
+    if (a.lds_cap < 0) return fail(CS_ERR_HIP, "%s", kDeviceQueryFailed);
     if (a.lds_cap == 0)
         return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
                     LightLayout<K>::bytes(LightLayout<K>::cap));
@@ -2766,6 +2771,7 @@ template <int K, bool F32>
 cs_status launch_light_sched(hipStream_t st, LightArgs a, const uint32_t *order, int64_t n_items,
                              unsigned long long *next) {
     a.lds_cap = light_capacity(K);
+    if (a.lds_cap < 0) return fail(CS_ERR_HIP, "%s", kDeviceQueryFailed);
     if (a.lds_cap == 0)
         return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",
                     LightLayout<K>::bytes(LightLayout<K>::cap));
@@ -3417,6 +3423,10 @@ extern "C" cs_status cs_cosine_batch_create(const cs_cosine_job *jobs, int32_t n
     b->bytes = b->off_scratch + need.scratch_bytes;
     // LDS attributes now, so that a run can be captured into a graph
     b->cap = light_capacity(b->K);
+    if (b->cap < 0) {
+        delete b;
+        return fail(CS_ERR_HIP, "%s", kDeviceQueryFailed);
+    }
     if (b->cap == 0) {
         delete b;
         return fail(CS_ERR_UNSUPPORTED, "device has less than %zu bytes of LDS per workgroup",

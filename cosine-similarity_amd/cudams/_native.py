@@ -134,6 +134,11 @@ def load(path: str = LIB_PATH):
         raise RuntimeError(
             f"{path} is not built; run `python cosine-similarity_amd/build.py` "
             "(hipcc --offload-arch=gfx950). There is no CPU fallback.")
+    # torch's wheel carries its own HIP runtime.  Loaded first, it is the one
+    # the library binds to (same soname); loaded after the library, it would
+    # be a second runtime in the process, and whichever opens the GPU second
+    # finds no device.
+    import torch  # noqa: F401
     L = ctypes.CDLL(path)
     p, i64, i32, sz = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_size_t
     L.cs_last_error.restype = ctypes.c_char_p
