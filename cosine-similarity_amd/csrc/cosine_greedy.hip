@@ -558,21 +558,28 @@ constexpr int kPool = 128;
 constexpr int kPoolSlots = kPool + 2;
 constexpr int kDropSlots = kTileQueries + 1;
 constexpr int kMidCap = 64;     // candidates a wave settles for one pair in registers
-constexpr int kBigPair = 512;   // heavy pairs above this many candidates are settled first
+// Heavy pairs settled by a whole workgroup go in four size classes, taken
+// largest first so that the largest pairs do not form the kernel's tail:
+// class 3 above 2048 candidates, 2 above 1024, 1 above 512, 0 the rest.
+constexpr int kHeavyClasses = 4;
+__device__ __forceinline__ int heavy_class(int n) {
+    return n > 2048 ? 3 : (n > 1024 ? 2 : (n > 512 ? 1 : 0));
+}
 // Heavy pairs of at most kWaveKeys candidates between spectra of at most
 // kWavePeaks peaks are settled by one wave each (eight pairs per heavy
 // workgroup at once) instead of by a whole workgroup.
 constexpr int kWavePeaks = 256;
 constexpr int kWaveKeys = 256;
 // Counters of one call (zeroed by k_fill_u64 before the lane kernel):
-// [0] heavy pairs, [1] next heavy item, [2] giant pairs, [3] next giant
-// item, [4] big heavy pairs, [5] [6] lane-kernel work items (scheduled /
-// batch launches), [7] small heavy pairs, [8] next small pair.  The heavy
-// queue's lower half holds the heavy pairs from its bottom up and the big
-// ones from its middle down; its upper half holds the small ones.
+// [1] next workgroup pair, [2] giant pairs, [3] next giant pair, [5] [6]
+// lane-kernel work items (scheduled / batch launches), [7] small pairs,
+// [8] next small pairs, [9 + k] workgroup pairs of size class k.  The heavy
+// queue's lower half holds four equal regions, one per size class, each
+// filled from its bottom; its upper half holds the small pairs.
 constexpr int kCounters = 16;
 constexpr int kSmallCount = 7;
 constexpr int kSmallNext = 8;
+constexpr int kClassCount = 9;
 // Slots of the block-relative reference offsets in LDS: kMaxLightRefs + 1
 // rounded up so that the pools after them start 16-byte aligned (their
 // float64 values are read and written as 8-byte LDS accesses).
@@ -1071,12 +1078,11 @@ __device__ void light_body(const LightArgs &a, int64_t rb, int64_t t) {
                 Q.off[q + 1] - Q.off[q] <= kWavePeaks) {  // one wave per pair
                 const unsigned long long slot = atomicAdd(&a.counters[kSmallCount], 1ull);
                 if ((int64_t)slot < a.heavy_cap - half) a.heavy_queue[half + slot] = e;
-            } else if (n > kBigPair) {  // from the middle down, taken first
-                const unsigned long long slot = atomicAdd(&a.counters[4], 1ull);
-                if ((int64_t)slot < half) a.heavy_queue[half - 1 - slot] = e;
-            } else {
-                const unsigned long long slot = atomicAdd(&a.counters[0], 1ull);
-                if ((int64_t)slot < half) a.heavy_queue[slot] = e;
+            } else {  // a workgroup per pair: the region of its size class
+                const int k = heavy_class(n);
+                const int64_t region = half / kHeavyClasses;
+                const unsigned long long slot = atomicAdd(&a.counters[kClassCount + k], 1ull);
+                if ((int64_t)slot < region) a.heavy_queue[k * region + slot] = e;
             }
             out_score = 0.0f;
             out_matches = -1;
@@ -2353,16 +2359,21 @@ __device__ __forceinline__ void heavy_body(const HeavyArgs &a, const HeavyArgs *
     __shared__ unsigned long long item_s;
     __shared__ int nlist;
     __shared__ long long list[kScanChunk];
-    // counters[4] counts the big pairs, queued from the middle of the queue
-    // downwards; they are taken first among the workgroup pairs so they do
-    // not form the tail.  The small pairs (the upper half) are settled one
-    // per wave before them.
-    const unsigned long long count = a.counters[0], big = a.counters[4],
-                             small = a.counters[kSmallCount];
-    if (count + big + small == 0) return;
+    // The small pairs (the queue's upper half) are settled one per wave
+    // first; then the workgroup pairs, largest size class first.
+    const int64_t half = a.heavy_cap >> 1, region = half / kHeavyClasses;
+    unsigned long long cls[kHeavyClasses], count = 0;
+    bool fits = true;
+#pragma unroll
+    for (int k = 0; k < kHeavyClasses; ++k) {
+        cls[k] = a.counters[kClassCount + k];
+        count += cls[k];
+        fits = fits && (int64_t)cls[k] <= region;
+    }
+    const unsigned long long small = a.counters[kSmallCount];
+    if (count + small == 0) return;
     const int tid = threadIdx.x;
-    const int64_t half = a.heavy_cap >> 1;
-    if ((int64_t)(count + big) <= half && (int64_t)small <= a.heavy_cap - half) {
+    if (fits && (int64_t)small <= a.heavy_cap - half) {
         if constexpr (STAGED) {
             // The small pairs first, one per wave: every resident wave takes
             // at most a few, so they form no tail.  One claim per workgroup
@@ -2393,10 +2404,22 @@ __device__ __forceinline__ void heavy_body(const HeavyArgs &a, const HeavyArgs *
             __syncthreads();
             const unsigned long long item = (unsigned long long)uniform64((int64_t)item_s);
             __syncthreads();
-            if (item >= count + big) break;
-            const unsigned long long e =
-                item < big ? a.heavy_queue[half - 1 - item] : a.heavy_queue[item - big];
-            const PairRef p = pair_of(a.batched, e);
+            if (item >= count) break;
+            // the item's class and place in it, classes 3 .. 0 in turn
+            static_assert(kHeavyClasses == 4, "four size classes");
+            unsigned long long idx = item;
+            int k;
+            if (idx < cls[3]) {
+                k = 3;
+            } else if ((idx -= cls[3]) < cls[2]) {
+                k = 2;
+            } else if ((idx -= cls[2]) < cls[1]) {
+                k = 1;
+            } else {
+                idx -= cls[1];
+                k = 0;
+            }
+            const PairRef p = pair_of(a.batched, a.heavy_queue[k * region + (int64_t)idx]);
             if constexpr (STAGED)
                 heavy_pair_keys(jobs ? jobs[p.job] : a, p.r, p.q, heavy_lds, &bn_s);
             else

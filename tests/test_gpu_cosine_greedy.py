@@ -531,16 +531,17 @@ def test_giant_pairs(gpu, oracle, T):
 
 @pytest.mark.parametrize("n,k", [(1040, 70), (730, 270)])
 def test_heavy_queue_overflow(gpu, n, k):
-    """More heavy pairs than their half of the 2^20-entry queue holds: n
+    """More heavy pairs than their region of the 2^20-entry queue holds: n
     spectra whose k peaks all sit at the same m/z (1 Da apart, tol 0.1), so
     every pair has exactly the k diagonal candidates (above the 64 the lane
     kernel settles).  k = 70: pairs settled one per wave (at most 256
-    candidates and peaks), k = 270: pairs settled by a workgroup each.  The
-    overflowing queue is abandoned for a scan of the matches plane; the
-    closed form is matches = k and score = cosine of the intensity vectors."""
+    candidates and peaks; their region holds 2^19), k = 270: pairs settled by
+    a workgroup each (their size class's region holds 2^17).  The overflowing
+    queue is abandoned for a scan of the matches plane; the closed form is
+    matches = k and score = cosine of the intensity vectors."""
     from cudams.spectra import PackedSpectra
 
-    assert n * n > (1 << 19)
+    assert n * n > (1 << 19 if k <= 256 else 1 << 17)
     rng = np.random.default_rng(21)
     mz = 100.0 + np.arange(k, dtype=np.float64)
     its = rng.random((n, k)).astype(np.float32).astype(np.float64) + 0.05
