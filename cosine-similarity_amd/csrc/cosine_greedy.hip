@@ -16,8 +16,10 @@
 //                        candidates go to the wave's LDS pool; pairs with at
 //                        most K candidates settle in registers, up to 64 by
 //                        the whole wave, more are queued
-//   4. k_heavy           one workgroup per queued pair: enumeration into LDS,
-//                        block bitonic sort by the (v desc, i desc, j desc)
+//   4. k_heavy           queued pairs: the small ones (at most 256 candidates
+//                        and peaks) one per wave, then one workgroup per
+//                        pair, largest size class first: enumeration into
+//                        LDS, bitonic sort by the (v desc, i desc, j desc)
 //                        key, one-wave greedy
 //   5. k_giant           one wave per pair above the heavy buffer, in rounds
 //                        of at most T candidates (exact in bounded LDS)
@@ -1301,7 +1303,7 @@ struct HeavyArgs {
     int64_t ld;
     const unsigned long long *heavy_queue;
     int64_t heavy_cap;
-    unsigned long long *counters;  // [0] heavy count [1] heavy next [2] giant count [3] giant next
+    unsigned long long *counters;  // the call's counters (kCounters of them, listed above)
     unsigned long long *giant_queue;
     int job;                       // this job's index in a batch
     int batched;                   // queue entries carry the job index
