@@ -20,7 +20,9 @@ host cores), ``parity`` (every pair against that oracle, near ties counted),
 ``fingerprint`` (the three measures at 4096 x 4096 x 2048 bits),
 ``batched_jobs`` (64 uneven jobs, graph vs one call per job) and ``serving``
 (500,000-spectrum library, Poisson request streams, served vs plain).
-``--workload fingerprint`` runs the fingerprint section as its own line.
+``--workload fingerprint`` runs the fingerprint section as its own line, and
+``--workload batched`` the 64-job batch's captured graph alone (a fixed launch
+shape, the one the autotuner picks in the default run; for profiling).
 """
 from __future__ import annotations
 
@@ -335,6 +337,43 @@ def bench_fingerprint(a) -> dict:
             "cpu_baseline": j.get("cpu_baseline"), "fingerprint": fp}
 
 
+def bench_batched_graph(a) -> dict:
+    """``--workload batched``: replays of the 64-job batch's captured graph
+    with the launch shape the default run's autotuner picks (K 8, round
+    capacity 256, 512 references per block), nothing else on the stream, so
+    a kernel trace of this run holds the batch kernels only."""
+    import torch
+
+    from cudams import synthetic
+    from cudams.batch import BatchParams, BatchRunner
+
+    jobs = synthetic.batched_jobs(seed=6)
+    pairs = sum(j.refs.n_spectra * j.queries.n_spectra for j in jobs)
+    runner = BatchRunner(jobs, BatchParams(8, 256, 512, 0))
+    runner.capture()
+    for _ in range(max(1, a.warmup)):
+        runner.replay()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(a.steps):
+        runner.replay()
+    e1.record(st)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    runner.close()
+    log(f"batched graph: {ms:.3f} ms per replay, {pairs} pairs")
+    return {"metric": "batched CosineGreedy pairs/s (64 uneven jobs, one graph)",
+            "value": pairs / (ms / 1e3), "unit": "pairs/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": max(1, a.warmup), "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic", "config": {"workload": "batched_jobs_64_seed6", "jobs": len(jobs),
+                                            "pairs": pairs, "mz_power": 0.5,
+                                            "tolerances": [0.01, 0.1]},
+            "roofline": None, "cpu_baseline": None}
+
+
 def bench_cpu_parallel(packed, tol, want) -> dict:
     """CPUParallelCosineGreedy (the host engine, libcsamd_cpu.so) on the whole
     matrix with this host's cores, wall clock from spectrum objects to the
@@ -567,7 +606,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--workload", choices=["cosine", "fingerprint", "copy"], default="cosine")
+    ap.add_argument("--workload", choices=["cosine", "fingerprint", "copy", "batched"],
+                    default="cosine")
     ap.add_argument("--light-candidates", type=int, default=0)
     ap.add_argument("--round-capacity", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -597,7 +637,8 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 1
-    res = {"cosine": bench_cosine, "fingerprint": bench_fingerprint, "copy": bench_copy}[a.workload](a)
+    res = {"cosine": bench_cosine, "fingerprint": bench_fingerprint, "copy": bench_copy,
+           "batched": bench_batched_graph}[a.workload](a)
     if a.workload == "cosine" and not a.no_fingerprint:
         res["fingerprint"] = bench_fingerprint_measures(a)
     if a.workload == "cosine" and not a.no_batched:
