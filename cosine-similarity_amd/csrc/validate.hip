@@ -1,5 +1,6 @@
-// validate.hip -- cs_validate_spectra: the CSR checks of cg_check.h run on
-// the device over a spectrum set the caller did not pack itself.
+// validate.hip -- device work on a spectrum set before it is compared:
+// cs_validate_spectra (the CSR checks of cg_check.h over a set the caller did
+// not pack itself) and cs_spectra_norms (the per-spectrum norms).
 #include "cg_check.h"
 #include "cs_internal.h"
 
@@ -25,6 +26,22 @@ __global__ void __launch_bounds__(kValidateThreads) k_validate(cs_spectra s, uin
     }
 }
 
+// norm[x] = sqrt(sum over x's peaks, in peak order, of (mz_pow * int_pow)^2)
+// (int_pow alone when mz_pow is NULL): one thread per spectrum, a sequential
+// float64 sum, so the value is the one a left-to-right host sum gives.
+__global__ void __launch_bounds__(kValidateThreads) k_norms(cs_spectra s, double *norm) {
+    const int64_t stride = (int64_t)gridDim.x * kValidateThreads;
+    for (int64_t x = blockIdx.x * (int64_t)kValidateThreads + threadIdx.x; x < s.n_spectra;
+         x += stride) {
+        double acc = 0.0;
+        for (int64_t p = s.offsets[x]; p < s.offsets[x + 1]; ++p) {
+            const double w = s.mz_pow ? s.mz_pow[p] * s.int_pow[p] : s.int_pow[p];
+            acc = acc + w * w;
+        }
+        norm[x] = sqrt(acc);
+    }
+}
+
 }  // namespace
 }  // namespace cs
 
@@ -45,5 +62,20 @@ extern "C" cs_status cs_validate_spectra(const cs_spectra *spectra, uint32_t *fa
     if (blocks > kValidateGrid) blocks = kValidateGrid;
     hipLaunchKernelGGL(k_validate, dim3((unsigned)blocks), dim3(kValidateThreads), 0, hs, s, fault);
     CS_LAUNCH_CHECK("k_validate");
+    return CS_OK;
+}
+
+extern "C" cs_status cs_spectra_norms(const cs_spectra *spectra, double *norm, void *stream) {
+    if (!spectra || !norm) return fail(CS_ERR_INVALID, "spectra/norm is NULL");
+    const cs_spectra &s = *spectra;
+    if (s.n_spectra < 0 || s.n_peaks < 0) return fail(CS_ERR_INVALID, "negative size");
+    if (s.n_spectra == 0) return CS_OK;
+    if (!s.offsets) return fail(CS_ERR_INVALID, "offsets must not be NULL");
+    if (s.n_peaks > 0 && !s.int_pow) return fail(CS_ERR_INVALID, "int_pow must not be NULL");
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    int64_t blocks = (s.n_spectra + kValidateThreads - 1) / kValidateThreads;
+    if (blocks > kValidateGrid) blocks = kValidateGrid;
+    hipLaunchKernelGGL(k_norms, dim3((unsigned)blocks), dim3(kValidateThreads), 0, hs, s, norm);
+    CS_LAUNCH_CHECK("k_norms");
     return CS_OK;
 }

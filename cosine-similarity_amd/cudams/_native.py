@@ -40,6 +40,7 @@ EXPORTED_SYMBOLS = (
     "cs_topk_rows",
     "cs_threshold_compact",
     "cs_validate_spectra",
+    "cs_spectra_norms",
     "cs_cosine_greedy_noalloc",
     "cs_pack_records",
 )
@@ -188,6 +189,8 @@ def load(path: str = LIB_PATH):
     L.cs_pack_records.argtypes = [p, p, p, i64, i64, i64, p, p]
     L.cs_validate_spectra.restype = i32
     L.cs_validate_spectra.argtypes = [ctypes.POINTER(CsSpectra), p, p]
+    L.cs_spectra_norms.restype = i32
+    L.cs_spectra_norms.argtypes = [ctypes.POINTER(CsSpectra), p, p]
     _lib = L
     return L
 

@@ -7,6 +7,7 @@ with are kept next to their float32 rounding.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 from typing import Iterable, Optional, Sequence, Tuple
 
@@ -98,20 +99,27 @@ class PackedSpectra:
             raise ValueError(f"a spectrum has more than {MAX_PEAKS} peaks; set n_max_peaks")
 
 
-def power_factors(p: PackedSpectra, mz_power: float, intensity_power: float):
-    """Per-peak float64 factors and per-spectrum norms, written as matchms
-    writes them: mz ** mz_power, intensities ** intensity_power and
-    sqrt(sum((mz**p * I**q) ** 2)) (matchms score_best_matches; the reference
-    GPU kernel computes the same norms at spectrum_similarity_functions.py:82-110).
-    mz_pow is None when mz_power == 0 (every factor is exactly 1.0)."""
+def peak_factors(p: PackedSpectra, mz_power: float, intensity_power: float):
+    """Per-peak float64 factors as matchms writes them: (mz ** mz_power, or
+    None when mz_power == 0, every factor then being exactly 1.0;
+    intensities ** intensity_power)."""
     mz_pow = None if mz_power == 0 else p.mz ** mz_power
     # x ** 1.0 is x exactly (IEEE pow); skip the pass for the default power
     int_pow = p.intensities if intensity_power == 1 else p.intensities ** intensity_power
+    return mz_pow, np.ascontiguousarray(int_pow, np.float64)
+
+
+def power_factors(p: PackedSpectra, mz_power: float, intensity_power: float):
+    """peak_factors and the per-spectrum norms sqrt(sum((mz**p * I**q) ** 2)),
+    summed left to right (matchms score_best_matches; the reference GPU kernel
+    computes the same norms at spectrum_similarity_functions.py:82-110).
+    cs_spectra_norms computes the same values on the device."""
+    mz_pow, int_pow = peak_factors(p, mz_power, intensity_power)
     w = int_pow if mz_pow is None else mz_pow * int_pow
     seg = np.repeat(np.arange(p.n_spectra), p.lengths)
     norm = np.sqrt(np.bincount(seg, weights=w * w, minlength=p.n_spectra)) if p.n_spectra else \
         np.zeros(0, np.float64)
-    return mz_pow, np.ascontiguousarray(int_pow, np.float64), norm.astype(np.float64)
+    return mz_pow, int_pow, norm.astype(np.float64)
 
 
 class DeviceSpectra:
@@ -124,7 +132,7 @@ class DeviceSpectra:
 
         self.packed = packed
         self.mz_power, self.intensity_power = float(mz_power), float(intensity_power)
-        mz_pow, int_pow, norm = power_factors(packed, mz_power, intensity_power)
+        mz_pow, int_pow = peak_factors(packed, mz_power, intensity_power)
         mz32 = packed.mz.astype(np.float32)
         exact = np.array_equal(mz32.astype(np.float64), packed.mz)
 
@@ -139,11 +147,19 @@ class DeviceSpectra:
         self.mz_exact = None if exact else up(packed.mz, np.float64)
         self.mz_pow = None if mz_pow is None else up(mz_pow, np.float64)
         self.int_pow = up(int_pow, np.float64)
-        self.norm = up(norm, np.float64)
         self.n_spectra = packed.n_spectra
         self.n_peaks = int(packed.offsets[-1])
         self.max_peaks = int(packed.lengths.max()) if self.n_spectra else 0
-        self.max_abs_mz = float(np.abs(packed.mz).max()) if self.n_peaks else 0.0
+        self.max_abs_mz = float(max(-packed.mz.min(), packed.mz.max())) if self.n_peaks else 0.0
+        # the norms on the device (cs_spectra_norms: the host's left-to-right
+        # sum), on the current stream after the uploads
+        from . import _native
+
+        self.norm = torch.empty(max(self.n_spectra, 1), dtype=torch.float64, device=device)
+        s = self.struct()
+        _native.check(_native.load().cs_spectra_norms(
+            ctypes.byref(s), self.norm.data_ptr(),
+            torch.cuda.current_stream(self.norm.device).cuda_stream))
 
     def slice(self, start: int, stop: int) -> "SpectraSlice":
         """Spectra [start, stop) as a view sharing this object's device memory."""

@@ -100,6 +100,18 @@ enum {
 cs_status cs_validate_spectra(const cs_spectra *spectra, uint32_t *fault, void *stream);
 
 /*
+ * norm[x] = sqrt(sum of (mz_pow[p] * int_pow[p])^2 over spectrum x's peaks,
+ * in peak order), int_pow[p]^2 when mz_pow is NULL: the norms that
+ * cs_spectra.norm must hold, computed on the device (`norm`: device
+ * [n_spectra] float64; spectra->norm is not read).  A sequential float64 sum
+ * per spectrum, as the host's left-to-right sum, so either may fill the
+ * field.  Asynchronous on `stream`.  Replaces the norm part of the reference
+ * kernel's preamble (spectrum_similarity_functions.py:82-110); the offsets
+ * are taken on trust (validate first for a set the caller did not pack).
+ */
+cs_status cs_spectra_norms(const cs_spectra *spectra, double *norm, void *stream);
+
+/*
  * CosineGreedy parameters (CudaCosineGreedy.py:24-56).  mz_power and
  * intensity_power are folded into cs_spectra.mz_pow / int_pow.
  */

@@ -123,6 +123,26 @@ def test_threshold_compact_matches_numpy(gpu):
     np.testing.assert_array_equal(oo, o[rr, cc])
 
 
+@pytest.mark.parametrize("mz_power,intensity_power", [(0.0, 1.0), (0.5, 1.0), (0.0, 0.5), (1.5, 2.0)])
+def test_device_norms_equal_host_norms(gpu, mz_power, intensity_power):
+    """cs_spectra_norms (DeviceSpectra's norms) equals the host's left-to-right
+    float64 sum bit for bit, empty spectra (norm 0) included."""
+    from cudams import synthetic
+    from cudams.spectra import DeviceSpectra, PackedSpectra, power_factors
+
+    p = synthetic.lognormal_spectra(3000, 17, cluster_fraction=0.2)
+    mzs = [p.mz[p.offsets[i]:p.offsets[i + 1]] for i in range(p.n_spectra)]
+    its = [p.intensities[p.offsets[i]:p.offsets[i + 1]] for i in range(p.n_spectra)]
+    for k in (0, 7, 2999):  # empty spectra at the ends and inside
+        mzs[k], its[k] = mzs[k][:0], its[k][:0]
+    p = PackedSpectra.from_arrays(mzs, its)
+    d = DeviceSpectra(p, mz_power, intensity_power, gpu)
+    want = power_factors(p, mz_power, intensity_power)[2]
+    got = d.norm.cpu().numpy()[:p.n_spectra]
+    assert got[0] == 0.0 and got[7] == 0.0 and got[2999] == 0.0
+    np.testing.assert_array_equal(got, want)
+
+
 @pytest.mark.parametrize("bad", ["none", "unsorted", "offsets_past_end", "offsets_decrease",
                                  "max_peaks", "max_abs_mz", "nan", "exact_unsorted"])
 def test_device_validator_flags_each_fault(gpu, bad):
