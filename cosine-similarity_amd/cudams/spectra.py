@@ -54,15 +54,14 @@ class PackedSpectra:
     def from_spectra(cls, spectra: Sequence, n_max_peaks: Optional[int] = None) -> "PackedSpectra":
         """Pack spectra, keeping the first ``n_max_peaks`` peaks of each (as the
         reference does at CudaCosineGreedy.py:82)."""
-        mzs, its = [], []
-        for s in spectra:
-            mz, it = spectrum_peaks(s)
-            if mz.shape != it.shape or mz.ndim != 1:
-                raise ValueError("m/z and intensity arrays must be 1-D of equal length")
-            if n_max_peaks is not None and len(mz) > n_max_peaks:
-                mz, it = mz[:n_max_peaks], it[:n_max_peaks]
-            mzs.append(mz)
-            its.append(it)
+        pairs = [spectrum_peaks(s) for s in spectra]
+        mzs = [mz for mz, _ in pairs]
+        its = [it for _, it in pairs]
+        if any(mz.ndim != 1 or it.ndim != 1 or len(mz) != len(it) for mz, it in pairs):
+            raise ValueError("m/z and intensity arrays must be 1-D of equal length")
+        if n_max_peaks is not None:
+            for k in np.flatnonzero(np.fromiter(map(len, mzs), np.int64, len(mzs)) > n_max_peaks):
+                mzs[k], its[k] = mzs[k][:n_max_peaks], its[k][:n_max_peaks]
         return cls._concat(mzs, its)  # already float64 1-D arrays
 
     @classmethod
@@ -93,8 +92,11 @@ class PackedSpectra:
             # a decrease of m/z is allowed only from a spectrum's last peak
             # to the next spectrum's first
             down = np.flatnonzero(self.mz[1:] < self.mz[:-1])
-            if down.size and not np.isin(down + 1, off[1:-1]).all():
-                raise ValueError("peaks must be sorted by m/z within each spectrum")
+            if down.size:
+                start = np.zeros(len(self.mz) + 1, dtype=bool)
+                start[off[1:-1]] = True
+                if not start[down + 1].all():
+                    raise ValueError("peaks must be sorted by m/z within each spectrum")
         if self.n_spectra and int(self.lengths.max()) > MAX_PEAKS:
             raise ValueError(f"a spectrum has more than {MAX_PEAKS} peaks; set n_max_peaks")
 
