@@ -501,8 +501,11 @@ def bench_serving(a) -> dict:
     # the workload's three rates, then a sustained rate between plain's and
     # served's saturation throughput (about 1,300 and 2,700 requests/s)
     for rate in (50.0, 200.0, 1000.0, 2000.0):
-        reqs, arr = stream(rate, a.serve_seconds, int(rate))
-        row = {"rate_per_s": rate, "seconds": a.serve_seconds}
+        # at least about 500 requests per rate, so that the p99 is not one of
+        # the few largest requests alone
+        seconds = max(a.serve_seconds, 500.0 / rate)
+        reqs, arr = stream(rate, seconds, int(rate))
+        row = {"rate_per_s": rate, "seconds": seconds}
         for mode in ("served", "plain"):
             r = run_load(mode, ix, reqs, arr)
             log(f"serving rate {rate}: {mode} p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms "
