@@ -159,9 +159,9 @@ class DeviceSpectra:
 
         self.norm = torch.empty(max(self.n_spectra, 1), dtype=torch.float64, device=device)
         s = self.struct()
-        _native.check(_native.load().cs_spectra_norms(
-            ctypes.byref(s), self.norm.data_ptr(),
-            torch.cuda.current_stream(self.norm.device).cuda_stream))
+        with torch.cuda.device(self.norm.device):
+            _native.check(_native.load().cs_spectra_norms(
+                ctypes.byref(s), self.norm.data_ptr(), torch.cuda.current_stream().cuda_stream))
 
     def slice(self, start: int, stop: int) -> "SpectraSlice":
         """Spectra [start, stop) as a view sharing this object's device memory."""
